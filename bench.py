@@ -1,0 +1,225 @@
+"""Throughput of PBRRenderer.step on one MI355X.
+
+Workload: the 64x64 CartPole scene with cart, pole and a shared ground quad
+(CartPoleGroundRenderer), 4096 scenes, uint8 RGB output [4096, 3, 64, 64].
+A step is one renderer.step(state): state [4096, 4] -> cart/pole instance
+transforms on the device -> one render kernel launch.  States are seeded
+(x ~ U(-2.4, 2.4), theta ~ U(-0.21, 0.21)) and resident on the GPU before
+timing starts.
+
+Printed JSON line (one, from this single process):
+  value        scene-frames/s over `--steps` timed steps (barrier +
+               synchronize on both sides)
+  roofline     render kernel: algorithmic bytes per launch (uint8 RGB pixels
+               + 80 B per dynamic instance + 64 B VP per scene) / average
+               kernel time from HIP events around back-to-back launches on
+               the launch stream, against 8 TB/s
+  cpu_baseline the CPU oracle (a C restatement of the same render path)
+               rendering a 256-scene sample of the same workload on all host
+               cores
+  env_step     CartPole-v0 (reference scene) env.step at the same size:
+               dynamics + auto-reset + render, scene-frames/s
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+PUBLISHED_ENV_STEP_L4 = 980562.0  # reference notebook, 4096 scenes 64x64, NVIDIA L4
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def seeded_states(n: int, batches: int, seed: int = 0):
+    import torch
+
+    g = torch.Generator("cpu").manual_seed(seed)
+    st = torch.zeros(batches, n, 4)
+    st[:, :, 0] = torch.empty(batches, n).uniform_(-2.4, 2.4, generator=g)
+    st[:, :, 2] = torch.empty(batches, n).uniform_(-0.21, 0.21, generator=g)
+    return st
+
+
+def algorithmic_bytes(r) -> int:
+    """Pixels written + per-scene instance rows read (80 B each) + VP (64 B)."""
+    N, C, H, W = r.output_shape()
+    esize = {"uint8": 1, "bfloat16": 2, "float32": 4}[r.cfg.output_dtype]
+    inst = sum(n.instances_per_scene for n in r._drawn_nodes() if not n.shared_across)
+    per_scene = H * W * C * esize + 80 * inst + 64
+    if r.cfg.output_depth:
+        per_scene += H * W * 4
+    return N * per_scene
+
+
+def kernel_time_us(r, launches: int = 200) -> float:
+    """Average render-kernel duration from HIP events bracketing back-to-back
+    launches of one prebuilt descriptor on the current stream."""
+    import torch
+
+    from pybatchrender_amd import _abi, _native
+
+    N, C, H, W = r.output_shape()
+    out = torch.empty((N, C, H, W), dtype=torch.uint8, device="cuda")
+    desc, keep = r.scene_descriptor()
+    fo = _abi.pbr_frame_out()
+    fo.color, fo.channels, fo.dtype = out.data_ptr(), C, _abi.PBR_DTYPE_U8
+    opts = r.render_opts()
+    lib = _native.library()
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    args = (ctypes.byref(desc), ctypes.byref(fo), ctypes.byref(opts), None, 0, sp)
+    for _ in range(20):
+        lib.pbr_render(*args)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(launches):
+        rc = lib.pbr_render(*args)
+        if rc:
+            raise RuntimeError(_native.last_error())
+    e1.record(stream)
+    e1.synchronize()
+    del keep
+    return e0.elapsed_time(e1) * 1000.0 / launches
+
+
+def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 10.0) -> dict:
+    """The CPU oracle on a bounded sample: the first `sample_scenes` scenes."""
+    from oracle import oracle
+
+    desc, keep = r.scene_descriptor()
+    hs = oracle.HostScene(desc, keep)
+    hs.desc.num_scenes = sample_scenes
+    import torch
+
+    from pybatchrender_amd import _abi
+
+    N, C, H, W = sample_scenes, 3, hs.desc.height, hs.desc.width
+    pix = torch.empty((N, C, H, W), dtype=torch.uint8)
+    fo = _abi.pbr_frame_out()
+    fo.color, fo.channels, fo.dtype = pix.data_ptr(), C, _abi.PBR_DTYPE_U8
+    threads = len(os.sched_getaffinity(0))
+    lib = oracle.library()
+    lib.pbr_oracle_render_mt(ctypes.byref(hs.desc), ctypes.byref(fo), threads)  # warmup
+    iters, t0 = 0, time.perf_counter()
+    while True:
+        lib.pbr_oracle_render_mt(ctypes.byref(hs.desc), ctypes.byref(fo), threads)
+        iters += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds and iters >= 3:
+            break
+    return {"value": N * iters / el, "unit": "scene-frames/s", "cores": threads, "kind": "port",
+            "sample": f"{N} scenes of the same workload (first {N} states), {iters} renders, "
+                      f"{el:.1f} s, C oracle on {threads} threads"}
+
+
+def env_step_rate(n: int, steps: int) -> float:
+    import torch
+
+    import pybatchrender_amd as pbr
+
+    env = pbr.envs.make("CartPole-v0", num_scenes=n)
+    td = env.reset()
+    for _ in range(3):
+        td["action"] = env.action_spec.rand()
+        td = env.step(td)["next"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        td["action"] = env.action_spec.rand()
+        td = env.step(td)["next"]
+    torch.cuda.synchronize()
+    return n * steps / (time.perf_counter() - t0)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--scenes", type=int, default=4096)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-env", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this renderer shards scenes with no "
+              "communication, so --gpus N > 1 is not run here.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    n = args.scenes
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda"))
+    states = seeded_states(n, args.warmup + args.steps).cuda()
+    log(f"[bench] warmup {args.warmup} steps, N={n}")
+    for i in range(args.warmup):
+        r.step(states[i])
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r.step(states[args.warmup + i])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = n * args.steps / el
+
+    log("[bench] kernel timing")
+    kus = kernel_time_us(r)
+    nbytes = algorithmic_bytes(r)
+    achieved = nbytes / (kus * 1e-6) / 1e9
+
+    rec = {
+        "metric": "scene-frames/s, 4096-env CartPole 64x64, 1 GPU; fwd+bwd vs plain fwd; HBM %",
+        "value": round(value, 1),
+        "unit": "scene-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el * 1000.0 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "CartPole cart+pole+ground quad, 64x64 uint8 RGB, renderer.step",
+                   "num_scenes": n, "tile": [64, 64], "channels": 3,
+                   "tris_per_scene": 26, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "render_kernel<64,false,0>", "kernel_us": round(kus, 3),
+                     "bytes_per_launch": nbytes},
+    }
+    if not args.no_env:
+        log("[bench] env.step CartPole-v0")
+        es = env_step_rate(n, 50)
+        rec["env_step"] = {"scene_frames_per_s": round(es, 1),
+                           "workload": "pbr.envs.make('CartPole-v0') env.step, reference scene",
+                           "published_l4": PUBLISHED_ENV_STEP_L4,
+                           "ratio_to_published": round(es / PUBLISHED_ENV_STEP_L4, 3)}
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle)")
+        rec["cpu_baseline"] = cpu_baseline(r, min_seconds=args.cpu_seconds)
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

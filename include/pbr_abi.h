@@ -1,0 +1,144 @@
+/*
+ * pbr_abi.h - C ABI of the MI355X batch renderer (libpbr_hip.so).
+ *
+ * One call renders N independent instanced scenes into a caller-owned
+ * [N, C, H, W] tensor.  It replaces, for the PBRRenderer.step() path of the
+ * reference (pybatchrender), the Panda3D instanced draw + GL raster + frame
+ * readback + atlas rearrange:
+ *
+ *   pybatchrender/renderer/renderer.py:377-389   PBRRenderer.step
+ *   pybatchrender/renderer/renderer.py:326-363   grab_pixels, _rearrange_img
+ *   pybatchrender/renderer/frame_grabber.py:85-106,122-167  readback + flip
+ *   pybatchrender/shaders/basic.vert:24-57        per-instance transform
+ *   pybatchrender/shaders/basic.frag:20-39        Lambert + ambient shading
+ *
+ * Conventions
+ *   - All pointers inside the descriptors are device pointers for
+ *     libpbr_hip.so.  The library never allocates: inputs, outputs and the
+ *     workspace belong to the caller (torch tensors in the Python host).
+ *   - Matrices are row-major float32 [4][4]; a vertex v transforms as
+ *     clip = VP * (M * v), the same product basic.vert:43 evaluates.
+ *   - Every entry point returns PBR_OK (0) or an error code; the message of
+ *     the last error on the calling thread is returned by pbr_last_error().
+ *   - The stream argument is a hipStream_t (NULL = legacy default stream).
+ *     Work is only enqueued; nothing synchronises, so calls can be captured
+ *     into a hipGraph.
+ */
+#ifndef PBR_ABI_H
+#define PBR_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBR_ABI_VERSION 1
+#define PBR_MAX_NODES 16
+#define PBR_MAX_MESHES 16
+/* Largest tile edge in pixels; bounds the fixed-point range (see DESIGN). */
+#define PBR_MAX_RESOLUTION 4096
+
+enum pbr_status {
+  PBR_OK = 0,
+  PBR_ERR_INVALID = 1,     /* bad descriptor / argument                   */
+  PBR_ERR_LAUNCH = 2,      /* kernel launch failed                        */
+  PBR_ERR_WORKSPACE = 3,   /* workspace missing or too small              */
+  PBR_ERR_UNSUPPORTED = 4  /* valid request the library cannot serve      */
+};
+
+enum pbr_dtype { PBR_DTYPE_U8 = 0, PBR_DTYPE_BF16 = 1, PBR_DTYPE_F32 = 2 };
+
+/* One mesh in the shared vertex/index pools (baked model space:
+ * node.py:61-71 loadModel + model scale/hpr + flattenStrong + pivot_to_rel). */
+typedef struct pbr_mesh {
+  int32_t vertex_offset;  /* first vertex of this mesh in scene.vertices   */
+  int32_t num_vertices;
+  int32_t index_offset;   /* first triangle of this mesh in scene.indices  */
+  int32_t num_triangles;
+} pbr_mesh;
+
+/* One instanced node (node.py:12-108, basic.vert:25-34,54).
+ * Instance i of scene s reads buffer row  b = shared ? i : s*ips + i. */
+typedef struct pbr_node {
+  const float* transforms;    /* [buf_instances][4][4] model matrices M     */
+  const float* colors;        /* [buf_instances][4] RGBA                    */
+  const int32_t* counts;      /* optional [num_scenes] active instances per
+                                 scene (NULL: all ips are drawn)            */
+  int32_t mesh;               /* index into pbr_scene_desc.meshes           */
+  int32_t instances_per_scene;
+  int32_t shared;             /* 1: buffers hold ips rows used by all scenes */
+  int32_t flat;               /* 1: every triangle's three vertex normals
+                                 are equal (faceted mesh, shade per face)   */
+} pbr_node;
+
+/* The batch: N scenes that share meshes and nodes but have their own
+ * instance transforms/colours and view-projection (camera.py:252-262). */
+typedef struct pbr_scene_desc {
+  int32_t num_scenes;
+  int32_t width;              /* tile resolution = output W                 */
+  int32_t height;             /* output H                                   */
+  int32_t num_nodes;
+  int32_t num_meshes;
+  int32_t reserved0;
+  const float* vertices;      /* [V][6] px py pz nx ny nz                   */
+  const int32_t* indices;     /* [T][3] vertex ids relative to the mesh     */
+  const float* view_proj;     /* [num_scenes][4][4] VP = P * V              */
+  float light_dir[3];         /* dirLightDir   (light.py:26-38)             */
+  float light_col[3];         /* dirLightCol                                */
+  float ambient[3];           /* ambientCol                                 */
+  float strength;             /* lightingStrength, clamped to [0,1]         */
+  float background[4];        /* clear colour RGBA                          */
+  pbr_mesh meshes[PBR_MAX_MESHES];
+  pbr_node nodes[PBR_MAX_NODES];
+} pbr_scene_desc;
+
+/* Outputs, all contiguous, written in place (no readback, no rearrange). */
+typedef struct pbr_frame_out {
+  void* color;                /* [N][channels][H][W] of dtype; row 0 = top  */
+  float* depth;               /* optional [N][H][W] window depth, 1 = clear */
+  int32_t* vis;               /* optional [N][H][W] winning triangle id of
+                                 the scene, -1 where nothing was drawn      */
+  int32_t channels;           /* 1..4: R, G, B, A (renderer.py:345 slice)   */
+  int32_t dtype;              /* enum pbr_dtype                             */
+} pbr_frame_out;
+
+/* Launch shape.  Zero fields take library defaults. */
+typedef struct pbr_render_opts {
+  int32_t region_rows;        /* output rows per work item (multiple of 8)  */
+  int32_t persistent;         /* 1: persistent (scene, region) work queue   */
+  int32_t workgroups;         /* persistent launch size, 0 = 8 per CU       */
+  int32_t reserved0;
+} pbr_render_opts;
+
+/* Render every scene of `scene` into `out` on `stream`.
+ * Replaces: renderer.py:377-389 (step: taskMgr.step + grab_pixels +
+ * _rearrange_img) together with shaders/basic.vert and shaders/basic.frag.
+ * `workspace` is device memory of at least pbr_workspace_bytes() bytes
+ * (may be NULL when that size is 0). */
+int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out,
+               const pbr_render_opts* opts, void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* Device workspace pbr_render needs for this batch and launch shape.
+ * No counterpart in the reference (GL owned its buffers). */
+int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts,
+                        size_t* bytes);
+
+/* Thread-local message of the last failed call ("" if none).
+ * Replaces the reference's swallowed exceptions (renderer.py:340-350). */
+const char* pbr_last_error(void);
+
+/* PBR_ABI_VERSION the library was built with. */
+int pbr_abi_version(void);
+
+/* Writes sizeof/offsetof facts of the descriptors into out[0..n) so a
+ * binding can verify its struct mirror; returns the number of facts. */
+int pbr_abi_layout(int64_t* out, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBR_ABI_H */

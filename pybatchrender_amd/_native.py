@@ -1,0 +1,51 @@
+"""Loader for libpbr_hip.so, the HIP renderer behind the C ABI.
+
+There is no fallback: if the library is missing or the device is not a GPU,
+rendering raises.  Build it with ``python -c "import __graft_entry__ as g;
+g.build()"`` (or ``pybatchrender_amd._build.build_hip()``).
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+from . import _abi
+
+LIB_PATH = Path(__file__).resolve().parent / "libpbr_hip.so"
+_LIB: ctypes.CDLL | None = None
+
+
+def library() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run pybatchrender_amd._build.build_hip() first"
+            )
+        lib = _abi.declare(ctypes.CDLL(str(LIB_PATH)))
+        _abi.check_layout(lib)
+        _LIB = lib
+    return _LIB
+
+
+def last_error() -> str:
+    msg = library().pbr_last_error()
+    return msg.decode() if msg else ""
+
+
+def render(desc: _abi.pbr_scene_desc, out: _abi.pbr_frame_out, opts: _abi.pbr_render_opts,
+           workspace_ptr: int, workspace_bytes: int, stream_ptr: int) -> None:
+    rc = library().pbr_render(
+        ctypes.byref(desc), ctypes.byref(out), ctypes.byref(opts),
+        ctypes.c_void_p(workspace_ptr or None), workspace_bytes, ctypes.c_void_p(stream_ptr or None),
+    )
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_render failed (status {rc}): {last_error()}")
+
+
+def workspace_bytes(desc: _abi.pbr_scene_desc, opts: _abi.pbr_render_opts) -> int:
+    n = ctypes.c_size_t(0)
+    rc = library().pbr_workspace_bytes(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(n))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_workspace_bytes failed (status {rc}): {last_error()}")
+    return int(n.value)

@@ -1,0 +1,305 @@
+"""PBRRenderer: the drop-in renderer object (pybatchrender/renderer/renderer.py:33-406).
+
+Same construction and scene API as the reference (add_node / add_camera /
+add_light / set_background_color / setup_environment / step / __call__ and
+the ``_step`` subclass hook), without ShowBase.  ``step`` enqueues one
+launch of the HIP kernel on torch's current stream; the kernel writes the
+final [N, C, H, W] tensor (no atlas, no readback, no rearrange).
+
+Observed reference behaviour kept on purpose:
+  * background: the reference's frames are cleared to Panda3D's default
+    window grey 0.41 (105 in uint8) although renderer.py:262 reads as black;
+    the default here is the observed (0.41, 0.41, 0.41, 1).
+  * no light added -> unlit (colour only), see light.py.
+  * projection aspect = whole atlas W/H (camera.py:154).
+  * depth test LESS, back faces culled, draw order node -> instance -> triangle.
+Deliberate difference: shared nodes with instances_per_scene > 1 draw every
+instance in every scene (the intended indexing of basic.vert:25-28); the
+reference's node.py:90 divides the buffer size by N and draws only instance 0.
+"""
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+from typing import Literal
+
+import numpy as np
+import torch
+
+from .. import _abi, _native
+from ..config import PBRConfig
+from .camera import PBRCam
+from .light import PBRLight
+from .mesh import Mesh
+from .node import PBRNode
+
+DEFAULT_BACKGROUND = (0.41, 0.41, 0.41, 1.0)
+_DTYPES = {
+    "uint8": (torch.uint8, _abi.PBR_DTYPE_U8),
+    "bfloat16": (torch.bfloat16, _abi.PBR_DTYPE_BF16),
+    "float32": (torch.float32, _abi.PBR_DTYPE_F32),
+}
+
+
+class PBRRenderer:
+    def __init__(self, cfg: PBRConfig | dict | None = None, **cfg_overrides):
+        self.cfg = PBRConfig.from_config(cfg, **cfg_overrides)
+        dev = torch.device(self.cfg.device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        self.num_scenes = int(self.cfg.num_scenes)
+        self._pbr_nodes: list[PBRNode] = []
+        self._pbr_cam: PBRCam | None = None
+        self._pbr_light: PBRLight | None = None
+        self._background_color = DEFAULT_BACKGROUND
+        self._geometry = None
+        self._workspace: torch.Tensor | None = None
+        self._env_ready = False
+
+    # --- scene construction --------------------------------------------------
+    def set_background_color(self, r: float, g: float, b: float, a: float = 1.0) -> None:
+        self._background_color = (float(r), float(g), float(b), float(a))
+
+    def add_node(
+        self,
+        model_path: "str | Mesh | None",
+        instances_per_scene: int,
+        texture=None,
+        model_pivot_relative_point: tuple[float, float, float] | None = None,
+        model_scale: float | Sequence[float] | None = None,
+        model_hpr: tuple[float, float, float] | None = None,
+        model_scale_units: Literal["relative", "absolute"] = "relative",
+        positions: torch.Tensor | None = None,
+        hprs: torch.Tensor | None = None,
+        scales: torch.Tensor | None = None,
+        colors: torch.Tensor | None = None,
+        backend: Literal["loop", "instanced"] = "instanced",
+        shared_across_scenes: bool = False,
+        parent: "PBRNode | None" = None,
+        name: str | None = None,
+    ) -> PBRNode:
+        node = PBRNode(
+            self, model_path=model_path, num_scenes=self.num_scenes,
+            instances_per_scene=int(instances_per_scene), texture=texture,
+            model_pivot_relative_point=model_pivot_relative_point, model_scale=model_scale,
+            model_hpr=model_hpr, model_scale_units=model_scale_units, positions=positions,
+            hprs=hprs, scales=scales, colors=colors, backend=backend,
+            shared_across_scenes=shared_across_scenes, parent=parent, name=name,
+        )
+        self._mark_geometry_dirty()
+        return node
+
+    def add_camera(self, fov_y_deg: float = 55.0, z_near: float = 0.05, z_far: float = 100.0,
+                   fixed_projection: bool = True) -> PBRCam:
+        self._pbr_cam = PBRCam(
+            self, num_scenes=self.cfg.num_scenes, cols=self.cfg.tiles[0], rows=self.cfg.tiles[1],
+            fov_y_deg=fov_y_deg, z_near=z_near, z_far=z_far, fixed_projection=fixed_projection,
+        )
+        return self._pbr_cam
+
+    def add_light(self, ambient=(0.2, 0.2, 0.25), dir_dir=(0.4, -0.6, -0.7), dir_col=(1.0, 1.0, 1.0),
+                  strength: float = 1.0) -> PBRLight:
+        self._pbr_light = PBRLight(self, ambient=ambient, dir_dir=dir_dir, dir_col=dir_col,
+                                   strength=strength)
+        return self._pbr_light
+
+    def _mark_geometry_dirty(self) -> None:
+        self._geometry = None
+
+    def setup_environment(self) -> None:
+        """Finish the scene: default camera if none (renderer.py:400-406) and
+        the one-time upload of the baked meshes."""
+        if self._pbr_cam is None:
+            self.add_camera()
+        self._build_geometry()
+        self._env_ready = True
+
+    # --- device geometry pool ----------------------------------------------------
+    def _drawn_nodes(self) -> list[PBRNode]:
+        return [n for n in self._pbr_nodes if n.has_geometry and n.buf_instances > 0
+                and n.mesh.num_triangles > 0]
+
+    def _build_geometry(self) -> None:
+        nodes = self._drawn_nodes()
+        if len(nodes) > _abi.PBR_MAX_NODES:
+            raise ValueError(f"at most {_abi.PBR_MAX_NODES} drawn nodes are supported")
+        blocks, tris, meshes = [], [], []
+        voff = toff = 0
+        for n in nodes:
+            m = n.mesh
+            blocks.append(m.vertex_block())
+            tris.append(m.triangles.astype(np.int32))
+            meshes.append((voff, m.num_vertices, toff, m.num_triangles))
+            voff += m.num_vertices
+            toff += m.num_triangles
+        if nodes:
+            v = torch.from_numpy(np.ascontiguousarray(np.concatenate(blocks)))
+            t = torch.from_numpy(np.ascontiguousarray(np.concatenate(tris)))
+        else:
+            v = torch.zeros((1, 6), dtype=torch.float32)
+            t = torch.zeros((1, 3), dtype=torch.int32)
+        self._geometry = {
+            "nodes": nodes,
+            "vertices": v.to(self.device).contiguous(),
+            "indices": t.to(self.device).contiguous(),
+            "meshes": meshes,
+            "flat": [n.flat for n in nodes],
+        }
+
+    # --- the C-ABI descriptor ------------------------------------------------------
+    def scene_descriptor(self) -> tuple[_abi.pbr_scene_desc, list[torch.Tensor]]:
+        """pbr_scene_desc of the current state plus the tensors it points to."""
+        if self._geometry is None:
+            self._build_geometry()
+        if self._pbr_cam is None:
+            self.add_camera()
+        g = self._geometry
+        N = self.num_scenes
+        H, W = int(self.cfg.tile_resolution[1]), int(self.cfg.tile_resolution[0])
+        d = _abi.pbr_scene_desc()
+        d.num_scenes, d.width, d.height = N, W, H
+        d.num_nodes = len(g["nodes"])
+        d.num_meshes = len(g["meshes"])
+        keep = [g["vertices"], g["indices"]]
+        d.vertices = g["vertices"].data_ptr()
+        d.indices = g["indices"].data_ptr()
+        vp = self._pbr_cam.VP_k44
+        if vp.shape != (N, 4, 4) or vp.dtype != torch.float32 or not vp.is_contiguous():
+            vp = vp.to(torch.float32).reshape(N, 4, 4).contiguous()
+        keep.append(vp)
+        d.view_proj = vp.data_ptr()
+        L = self._pbr_light
+        if L is None:
+            d.strength = 0.0
+        else:
+            d.light_dir[:] = L.dir_dir
+            d.light_col[:] = L.dir_col
+            d.ambient[:] = L.ambient
+            d.strength = L.strength
+        d.background[:] = self._background_color
+        for k, (voff, nv, toff, nt) in enumerate(g["meshes"]):
+            m = d.meshes[k]
+            m.vertex_offset, m.num_vertices, m.index_offset, m.num_triangles = voff, nv, toff, nt
+        for k, node in enumerate(g["nodes"]):
+            rows = node.instances_per_scene if node.shared_across else N * node.instances_per_scene
+            tr = node.transforms_b44
+            co = node.colors_b4
+            if tr.dtype != torch.float32 or not tr.is_contiguous():
+                tr = tr.to(torch.float32).contiguous()
+            if co.dtype != torch.float32 or not co.is_contiguous():
+                co = co.to(torch.float32).contiguous()
+            if tr.shape[0] < rows or tr.shape[1:] != (4, 4) or co.shape[0] < rows or co.shape[1] != 4:
+                raise ValueError(
+                    f"node {node.name}: needs {rows} transform/colour rows, has "
+                    f"{tuple(tr.shape)} / {tuple(co.shape)}")
+            if tr.device != self.device or co.device != self.device:
+                raise ValueError(f"node {node.name}: buffers are not on {self.device}")
+            keep += [tr, co]
+            nd = d.nodes[k]
+            nd.transforms, nd.colors = tr.data_ptr(), co.data_ptr()
+            nd.counts = node.instance_counts.data_ptr() if node.instance_counts is not None else None
+            if node.instance_counts is not None:
+                keep.append(node.instance_counts)
+            nd.mesh = k
+            nd.instances_per_scene = node.instances_per_scene
+            nd.shared = 1 if node.shared_across else 0
+            nd.flat = 1 if g["flat"][k] else 0
+        return d, keep
+
+    # --- rendering --------------------------------------------------------------------
+    def output_shape(self) -> tuple[int, int, int, int]:
+        W, H = self.cfg.tile_resolution
+        return (self.num_scenes, int(self.cfg.num_channels), int(H), int(W))
+
+    def render(self, out: torch.Tensor | None = None, depth: torch.Tensor | bool | None = None,
+               vis: torch.Tensor | bool | None = None) -> dict:
+        """Render the current scene state.  Returns {'pixels', 'depth'?, 'vis'?};
+        pass preallocated tensors to render in place (graph capture)."""
+        if self.device.type != "cuda":
+            raise RuntimeError(
+                f"the HIP renderer needs a GPU device (cfg.device='{self.cfg.device}')")
+        N, C, H, W = self.output_shape()
+        tdtype, code = _DTYPES[self.cfg.output_dtype]
+        if out is None:
+            out = torch.empty((N, C, H, W), dtype=tdtype, device=self.device)
+        elif out.shape != (N, C, H, W) or out.dtype != tdtype or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous {tdtype} tensor of shape {(N, C, H, W)}")
+        if depth is None:
+            depth = bool(self.cfg.output_depth)
+        if depth is True:
+            depth = torch.empty((N, H, W), dtype=torch.float32, device=self.device)
+        if vis is True:
+            vis = torch.empty((N, H, W), dtype=torch.int32, device=self.device)
+        for t, dt in ((depth, torch.float32), (vis, torch.int32)):
+            if isinstance(t, torch.Tensor) and (t.shape != (N, H, W) or t.dtype != dt
+                                                 or not t.is_contiguous()):
+                raise ValueError(f"depth/vis must be contiguous {dt} tensors of shape {(N, H, W)}")
+        desc, keep = self.scene_descriptor()
+        fo = _abi.pbr_frame_out()
+        fo.color = out.data_ptr()
+        fo.depth = depth.data_ptr() if isinstance(depth, torch.Tensor) else None
+        fo.vis = vis.data_ptr() if isinstance(vis, torch.Tensor) else None
+        fo.channels = C
+        fo.dtype = code
+        opts = self.render_opts()
+        ws_bytes = _native.workspace_bytes(desc, opts)
+        ws_ptr = 0
+        if ws_bytes:
+            if self._workspace is None or self._workspace.numel() < ws_bytes:
+                self._workspace = torch.zeros(ws_bytes, dtype=torch.uint8, device=self.device)
+            ws_ptr = self._workspace.data_ptr()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _native.render(desc, fo, opts, ws_ptr, ws_bytes, stream)
+        del keep
+        result = {"pixels": out}
+        if isinstance(depth, torch.Tensor):
+            result["depth"] = depth
+        if isinstance(vis, torch.Tensor):
+            result["vis"] = vis
+        return result
+
+    def render_opts(self) -> _abi.pbr_render_opts:
+        o = _abi.pbr_render_opts()
+        o.region_rows = int(self.cfg.region_rows)
+        o.persistent = 1 if self.cfg.persistent else 0
+        o.workgroups = 0
+        return o
+
+    def _step(self, *args, **kwargs):
+        """Subclass hook: map the caller's state into node/camera tensors."""
+        return None
+
+    def step(self, *args, return_pixels: bool = True, **kwargs):
+        """renderer.py:377-389: update the scene via ``_step``, then render."""
+        self._step(*args, **kwargs)
+        if return_pixels:
+            return self.render()["pixels"]
+        return None
+
+    def __call__(self, *args, return_pixels: bool = True, **kwargs):
+        return self.step(*args, return_pixels=return_pixels, **kwargs)
+
+    def grab_pixels(self) -> torch.Tensor:
+        """The current frame in the reference's atlas layout [H_win, W_win, C]
+        (row 0 = top), i.e. the inverse of ``_rearrange_img``."""
+        img = self.render()["pixels"]
+        cols, rows = self.cfg.tiles
+        N, C, H, W = img.shape
+        full = torch.zeros((cols * rows, C, H, W), dtype=img.dtype, device=img.device)
+        full[:N] = img
+        return full.view(rows, cols, C, H, W).permute(0, 3, 1, 4, 2).reshape(rows * H, cols * W, C)
+
+    def _rearrange_img(self, img: torch.Tensor) -> torch.Tensor:
+        """Atlas [H_win, W_win, C] -> [N, C, H, W] (renderer.py:352-363)."""
+        C = self.cfg.num_channels
+        img = img.permute(2, 0, 1)
+        img = img.reshape(C, self.cfg.tiles[1], self.cfg.tile_resolution[1], self.cfg.tiles[0],
+                          self.cfg.tile_resolution[0])
+        img = img.permute(1, 3, 0, 2, 4)
+        img = img.reshape(-1, C, self.cfg.tile_resolution[1], self.cfg.tile_resolution[0])
+        return img[: self.cfg.num_scenes]
+
+    def destroy(self) -> None:
+        self._geometry = None
+        self._workspace = None

@@ -1,0 +1,62 @@
+"""The C-ABI library loads on a host without a GPU and exports its header."""
+import ctypes
+import re
+from pathlib import Path
+
+from pybatchrender_amd import _abi, _native
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions() -> set[str]:
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names |= set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(pbr_[a-z_0-9]+)\s*\(", text, re.M))
+    return names
+
+
+def test_header_declares_the_exports():
+    assert declared_functions() == set(_abi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_native.LIB_PATH))
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layout_matches_header():
+    lib = _native.library()
+    _abi.check_layout(lib)
+    assert lib.pbr_abi_version() == _abi.PBR_ABI_VERSION
+
+
+def test_invalid_descriptor_is_rejected_without_gpu():
+    """Validation runs before any HIP call, so errors surface on any host."""
+    lib = _native.library()
+    d = _abi.pbr_scene_desc()
+    d.num_scenes, d.width, d.height = 1, 0, 64
+    fo = _abi.pbr_frame_out()
+    fo.channels, fo.dtype = 3, _abi.PBR_DTYPE_U8
+    rc = lib.pbr_render(ctypes.byref(d), ctypes.byref(fo), None, None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID
+    assert "resolution" in _native.last_error()
+    d.width = 64
+    fo.channels = 7
+    assert lib.pbr_render(ctypes.byref(d), ctypes.byref(fo), None, None, 0, None) == _abi.PBR_ERR_INVALID
+    assert "channels" in _native.last_error()
+    fo.channels = 3
+    fo.color = d.view_proj = 64  # never dereferenced: validation fails first
+    d.num_nodes = 1
+    d.num_meshes = 0
+    assert lib.pbr_render(ctypes.byref(d), ctypes.byref(fo), None, None, 0, None) == _abi.PBR_ERR_INVALID
+    assert "mesh" in _native.last_error()
+
+
+def test_workspace_size():
+    d = _abi.pbr_scene_desc()
+    o = _abi.pbr_render_opts()
+    assert _native.workspace_bytes(d, o) == 0
+    o.persistent = 1
+    assert _native.workspace_bytes(d, o) >= 4

@@ -1,0 +1,219 @@
+"""Host-side logic on the CPU: config rules, scene math, node/camera state,
+descriptor packing, the registry and the env loop without pixels.
+
+Closed forms restated from the reference: rotation R = Rz(h) Ry(p) Rx(r)
+(shader_context.py:48-84), GL perspective (camera.py:150-166), look-at rows
+(camera.py:201-240), atlas tiling (config.py:61-126), instance rows
+(node.py:45-46, basic.vert:25-28).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import pybatchrender_amd as pbr
+from pybatchrender_amd import PBRConfig, PBRRenderer
+from pybatchrender_amd.renderer import mesh as meshes
+from pybatchrender_amd.renderer.shader_context import PBRShaderContext
+
+
+# --- config -----------------------------------------------------------------------
+@pytest.mark.parametrize("n,tiles", [(1, (1, 1)), (4, (2, 2)), (5, (3, 2)), (4096, (64, 64)),
+                                     (4098, (65, 64)), (16384, (128, 128))])
+def test_tiles_from_num_scenes(n, tiles):
+    cfg = PBRConfig(num_scenes=n, tile_resolution=(64, 64), device="cpu")
+    assert cfg.tiles == tiles
+    assert cfg.window_resolution == (64 * tiles[0], 64 * tiles[1])
+
+
+def test_config_errors():
+    with pytest.raises(ValueError):
+        PBRConfig(tiles=(2, 2), tile_resolution=(64, 64), window_resolution=(100, 128), device="cpu")
+    with pytest.raises(ValueError):
+        PBRConfig(num_scenes=10, tiles=(2, 2), device="cpu")
+    with pytest.raises(ValueError):
+        PBRConfig(num_scenes=1, device="tpu")
+    with pytest.raises(ValueError):
+        PBRConfig(num_scenes=1, device="cpu", output_dtype="int16")
+    cfg = PBRConfig.from_config({"num_scenes": 3, "device": "cpu"}, tile_resolution=(32, 16))
+    assert cfg.window_resolution == (64, 32)
+
+
+# --- math closed forms ---------------------------------------------------------------
+def test_rotation_convention():
+    h, p, r = 0.3, -0.7, 1.1
+    R = PBRShaderContext._rotation_mats_from_hpr(torch.tensor([[h, p, r]])).numpy()[0]
+    cz, sz, cy, sy, cx, sx = math.cos(h), math.sin(h), math.cos(p), math.sin(p), math.cos(r), math.sin(r)
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    assert np.allclose(R, Rz @ Ry @ Rx, atol=1e-6)
+    # pitch alone rotates +z towards +x (the pole tilt of CartPole)
+    Rp = PBRShaderContext._rotation_mats_from_hpr(torch.tensor([[0.0, 0.5, 0.0]])).numpy()[0]
+    assert np.allclose(Rp @ [0, 0, 1], [math.sin(0.5), 0, math.cos(0.5)], atol=1e-6)
+
+
+def test_projection_and_view():
+    r = PBRRenderer(PBRConfig(num_scenes=6, tile_resolution=(64, 32), device="cpu"))
+    cam = r.add_camera(fov_y_deg=60.0, z_near=0.5, z_far=20.0)
+    P = cam.P_k44.numpy()
+    aspect = r.cfg.window_resolution[0] / r.cfg.window_resolution[1]  # whole atlas (camera.py:154)
+    f = 1 / math.tan(math.radians(30))
+    assert np.isclose(P[0, 0], f / aspect) and np.isclose(P[1, 1], f)
+    assert np.isclose(P[2, 2], 20.5 / -19.5) and np.isclose(P[2, 3], 20.0 / -19.5) and P[3, 2] == -1
+    eye = torch.tensor([[1.0, -5.0, 2.0]]).repeat(6, 1)
+    eye[:, 0] += torch.arange(6)
+    cam.set_positions_and_lookat(eye, torch.zeros(3))
+    V = cam.V_k44.numpy()
+    for k in range(6):
+        R3 = V[k, :3, :3]
+        assert np.allclose(R3 @ R3.T, np.eye(3), atol=1e-5)
+        e = np.append(eye[k].numpy(), 1.0)
+        assert np.allclose(V[k] @ e, [0, 0, 0, 1], atol=1e-5)  # eye -> origin
+        tgt = V[k] @ np.array([0, 0, 0, 1.0])
+        assert tgt[2] < 0 and abs(tgt[0]) < 1e-5 and abs(tgt[1]) < 1e-5  # target on -z axis
+    assert np.allclose(cam.VP_k44.numpy(), np.einsum("ij,bjk->bik", P, V), atol=1e-5)
+
+
+def test_set_projection_takes_effect_at_next_view_update():
+    r = PBRRenderer(PBRConfig(num_scenes=1, device="cpu"))
+    cam = r.add_camera()
+    vp0 = cam.get_vp()
+    cam.set_projection(fov_y_deg=30.0)
+    assert torch.equal(cam.get_vp(), vp0)
+    cam.set_eye(cam.get_eye())
+    assert not torch.equal(cam.get_vp(), vp0)
+
+
+# --- meshes ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name,nv,nt", [("models/box", 24, 12), ("models/quad", 4, 2),
+                                        ("models/sphere", 114, 224), ("models/cylinder", 130, 128)])
+def test_builtin_meshes_are_outward_ccw(name, nv, nt):
+    m = meshes.load_model(name)
+    assert (m.num_vertices, m.num_triangles) == (nv, nt)
+    P, N, T = m.positions, m.normals, m.triangles
+    face = np.cross(P[T[:, 1]] - P[T[:, 0]], P[T[:, 2]] - P[T[:, 0]])
+    assert (np.einsum("ij,ij->i", face, N[T].mean(1)) > 0).all()
+
+
+def test_box_is_flat_sphere_is_not():
+    assert meshes.box().is_flat()
+    assert not meshes.uv_sphere().is_flat()
+
+
+def test_bake_scale_and_pivot():
+    m = meshes.bake(meshes.box(), model_scale=(1.2, 0.8, 0.5), pivot_relative_point=(0.5, 0.5, 0.05))
+    assert np.allclose(m.positions.min(0), [-0.6, -0.4, -0.025])
+    assert np.allclose(m.positions.max(0), [0.6, 0.4, 0.475])
+    a = meshes.bake(meshes.box(), model_scale=2.0, model_scale_units="absolute")
+    assert np.allclose(a.positions.max(0) - a.positions.min(0), [2, 2, 2])
+    b = meshes.bake(meshes.box(), model_scale=(4.0, 1.0, 2.0), model_scale_units="absolute")
+    assert np.allclose(b.positions.max(0) - b.positions.min(0), [4, 1, 2])
+    with pytest.raises(ValueError):
+        meshes.bake(meshes.box(), model_scale=(1.0, 2.0))
+
+
+# --- nodes and the descriptor ----------------------------------------------------------
+def test_node_setters_match_reference_formula():
+    r = PBRRenderer(PBRConfig(num_scenes=3, device="cpu"))
+    node = r.add_node("models/box", instances_per_scene=2)
+    assert node.buf_instances == 6
+    pos = torch.arange(18, dtype=torch.float32).reshape(6, 3)
+    hpr = torch.rand(6, 3)
+    node.set_positions(pos, lazy=True)
+    node.set_hprs(hpr, lazy=True)
+    node.set_scales(torch.full((6,), 2.0))
+    M = node.transforms_b44
+    R = PBRShaderContext._rotation_mats_from_hpr(hpr)
+    assert torch.allclose(M[:, :3, :3], R * 2.0)
+    assert torch.equal(M[:, :3, 3], pos)
+    assert torch.equal(M[:, 3], torch.tensor([0.0, 0, 0, 1]).expand(6, 4))
+    mats = M.clone()
+    node.set_transforms(mats)
+    assert torch.allclose(node.transforms_b44, mats, atol=1e-6)
+
+
+def test_shared_node_rows_and_counts_in_descriptor():
+    r = PBRRenderer(PBRConfig(num_scenes=4, device="cpu"))
+    shared = r.add_node("models/quad", instances_per_scene=3, shared_across_scenes=True)
+    per = r.add_node("models/sphere", instances_per_scene=5)
+    per.set_instance_counts(torch.tensor([0, 5, 9, 2]))
+    r.setup_environment()
+    d, keep = r.scene_descriptor()
+    assert d.num_nodes == 2 and d.num_scenes == 4
+    assert (d.nodes[0].shared, d.nodes[0].instances_per_scene, d.nodes[0].flat) == (1, 3, 1)
+    assert (d.nodes[1].shared, d.nodes[1].instances_per_scene, d.nodes[1].flat) == (0, 5, 0)
+    assert d.meshes[1].num_triangles == 224 and d.meshes[1].index_offset == 2
+    assert per.instance_counts.tolist() == [0, 5, 5, 2]
+    assert d.nodes[1].counts == per.instance_counts.data_ptr()
+    assert d.strength == 0.0  # no light -> unlit
+    assert tuple(round(v, 4) for v in d.background) == (0.41, 0.41, 0.41, 1.0)
+
+
+def test_too_few_rows_rejected():
+    r = PBRRenderer(PBRConfig(num_scenes=4, device="cpu"))
+    node = r.add_node("models/box", instances_per_scene=1)
+    node.transforms_b44 = node.transforms_b44[:2].clone()
+    with pytest.raises(ValueError):
+        r.scene_descriptor()
+
+
+def test_render_on_cpu_device_fails_loudly():
+    r = PBRRenderer(PBRConfig(num_scenes=2, device="cpu"))
+    r.add_node("models/box", instances_per_scene=1)
+    r.setup_environment()
+    with pytest.raises(RuntimeError, match="GPU"):
+        r.step()
+
+
+def test_atlas_rearrange_roundtrip():
+    r = PBRRenderer(PBRConfig(num_scenes=5, tile_resolution=(4, 3), device="cpu"))
+    img = torch.arange(5 * 3 * 3 * 4, dtype=torch.int32).reshape(5, 3, 3, 4)
+    cols, rows = r.cfg.tiles
+    full = torch.zeros((cols * rows, 3, 3, 4), dtype=img.dtype)
+    full[:5] = img
+    atlas = full.view(rows, cols, 3, 3, 4).permute(0, 3, 1, 4, 2).reshape(rows * 3, cols * 4, 3)
+    assert torch.equal(r._rearrange_img(atlas), img)
+
+
+# --- registry and env --------------------------------------------------------------------
+def test_registry_surface():
+    assert "CartPole-v0" in pbr.envs.list_envs()
+    E, R, C = pbr.envs.get_env_classes("CartPole-v0")
+    assert R.__name__ == "CartPoleRenderer" and C.__name__ == "CartPoleConfig"
+    with pytest.raises(ValueError):
+        pbr.envs.make("Nope-v0")
+
+
+def test_cartpole_env_without_pixels_on_cpu():
+    env = pbr.envs.make("CartPole-v0", num_scenes=32, device="cpu", render=False)
+    td = env.reset()
+    assert td["observation"].shape == (32, 4)
+    for _ in range(5):
+        td["action"] = env.action_spec.rand()
+        td = env.step(td)["next"]
+    assert td["reward"].shape == (32, 1) and "pixels" not in td
+
+
+def test_cartpole_env_drops_pixels_when_render_fails_on_cpu():
+    """Reference behaviour (cartpole/env.py:171-176,191-196): render errors are
+    swallowed by the env; the renderer itself raises."""
+    env = pbr.envs.make("CartPole-v0", num_scenes=8, device="cpu")
+    td = env.reset()
+    assert "pixels" not in td
+
+
+def test_cartpole_dynamics_closed_form():
+    env = pbr.envs.make("CartPole-v0", num_scenes=2, device="cpu", render=False)
+    obs = torch.tensor([[0.1, -0.2, 0.05, 0.3], [0.0, 0.0, 0.0, 0.0]])
+    nxt = env._dynamics(obs, torch.tensor([1, 0]))
+    x, xd, th, thd = obs[0].tolist()
+    g, mc, mp, l, F, tau = 9.8, 1.0, 0.1, 0.5, 10.0, 0.02
+    temp = (F + mp * l * thd ** 2 * math.sin(th)) / (mc + mp)
+    tacc = (g * math.sin(th) - math.cos(th) * temp) / (l * (4 / 3 - mp * math.cos(th) ** 2 / (mc + mp)))
+    xacc = temp - mp * l * tacc * math.cos(th) / (mc + mp)
+    want = [x + tau * xd, xd + tau * xacc, th + tau * thd, thd + tau * tacc]
+    assert np.allclose(nxt[0].numpy(), want, atol=1e-6)
