@@ -3,17 +3,19 @@
 Workload: the 64x64 CartPole scene with cart, pole and a shared ground quad
 (CartPoleGroundRenderer), 4096 scenes, uint8 RGB output [4096, 3, 64, 64].
 A step is one renderer.step(state): state [4096, 4] -> cart/pole instance
-transforms on the device -> one render kernel launch.  States are seeded
-(x ~ U(-2.4, 2.4), theta ~ U(-0.21, 0.21)) and resident on the GPU before
-timing starts.
+transforms on the device (one pose launch) -> setup kernel -> raster kernel.
+States are seeded (x ~ U(-2.4, 2.4), theta ~ U(-0.21, 0.21)), one batch per
+step, and resident on the GPU before timing starts.
 
 Printed JSON line (one, from this single process):
-  value        scene-frames/s over `--steps` timed steps (barrier +
-               synchronize on both sides)
-  roofline     render kernel: algorithmic bytes per launch (uint8 RGB pixels
-               + 80 B per dynamic instance + 64 B VP per scene) / average
-               kernel time from HIP events around back-to-back launches on
-               the launch stream, against 8 TB/s
+  value        scene-frames/s over `--steps` timed renderer.step calls
+               (synchronize on both sides); `graphed_step` is the same step
+               captured once as a hipGraph and replayed (it adds the copy of
+               each state batch into the graph's static input)
+  roofline     the render call (setup kernel + raster kernel): algorithmic
+               bytes per call (uint8 RGB pixels + 80 B per dynamic instance +
+               64 B VP per scene) / average time of the call from HIP events
+               around back-to-back calls on the launch stream, vs 8 TB/s
   cpu_baseline the CPU oracle (a C restatement of the same render path)
                rendering a 256-scene sample of the same workload on all host
                cores
@@ -76,9 +78,12 @@ def kernel_time_us(r, launches: int = 200) -> float:
     fo.color, fo.channels, fo.dtype = out.data_ptr(), C, _abi.PBR_DTYPE_U8
     opts = r.render_opts()
     lib = _native.library()
+    ws_bytes = _native.workspace_bytes(desc, opts)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
-    args = (ctypes.byref(desc), ctypes.byref(fo), ctypes.byref(opts), None, 0, sp)
+    args = (ctypes.byref(desc), ctypes.byref(fo), ctypes.byref(opts), ctypes.c_void_p(ws.data_ptr()),
+            ws_bytes, sp)
     for _ in range(20):
         lib.pbr_render(*args)
     torch.cuda.synchronize()
@@ -93,6 +98,31 @@ def kernel_time_us(r, launches: int = 200) -> float:
     e1.synchronize()
     del keep
     return e0.elapsed_time(e1) * 1000.0 / launches
+
+
+def measured_traffic() -> int | None:
+    """HBM bytes per render call of this workload from the committed counter
+    profile (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
+    prof = ROOT / "profiles" / "render_cartpole_ground_4096x64x64_u8.md"
+    if not prof.exists():
+        return None
+    for line in prof.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_call:"):
+            return int(line.split(":")[1].strip())
+    return None
+
+
+def copy_calibration(mib: int = 256, reps: int = 5) -> None:
+    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counter
+    values calibrate the render kernel's traffic figures."""
+    import torch
+
+    src = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": mib << 20, "reps": reps}), flush=True)
 
 
 def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 10.0) -> dict:
@@ -153,6 +183,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-env", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the back-to-back render launches (for counter profiles)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="with --kernel-only: also copy 256 MiB device-to-device 5 times")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this renderer shards scenes with no "
@@ -169,17 +203,34 @@ def main() -> int:
     n = args.scenes
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda"))
     states = seeded_states(n, args.warmup + args.steps).cuda()
-    log(f"[bench] warmup {args.warmup} steps, N={n}")
+    if args.kernel_only:
+        log("[bench] kernel timing only")
+        kus = kernel_time_us(r, launches=args.steps)
+        print(json.dumps({"kernel_us": kus, "bytes_per_launch": algorithmic_bytes(r)}), flush=True)
+        if args.copy_calibration:
+            copy_calibration()
+        return 0
+    log(f"[bench] eager: warmup {args.warmup} steps, N={n}")
     for i in range(args.warmup):
         r.step(states[i])
     torch.cuda.synchronize()
-    log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for i in range(args.steps):
         r.step(states[args.warmup + i])
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     value = n * args.steps / el
+
+    log("[bench] graphed steps")
+    g = r.capture_step(states[0])
+    for i in range(args.warmup):
+        g(states[i])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g(states[args.warmup + i])
+    torch.cuda.synchronize()
+    graphed = n * args.steps / (time.perf_counter() - t0)
 
     log("[bench] kernel timing")
     kus = kernel_time_us(r)
@@ -203,10 +254,13 @@ def main() -> int:
                    "num_scenes": n, "tile": [64, 64], "channels": 3,
                    "tris_per_scene": 26, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "render_kernel<64,false,0>", "kernel_us": round(kus, 3),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(),
+                     "kernel": "setup_kernel<false> + raster_kernel<8,4,false,0>", "kernel_us": round(kus, 3),
                      "bytes_per_launch": nbytes},
     }
+    rec["graphed_step"] = {"scene_frames_per_s": round(graphed, 1),
+                           "note": "PBRRenderer.capture_step replay incl. state copy"}
     if not args.no_env:
         log("[bench] env.step CartPole-v0")
         es = env_step_rate(n, 50)

@@ -105,12 +105,42 @@ typedef struct pbr_frame_out {
 } pbr_frame_out;
 
 /* Launch shape.  Zero fields take library defaults. */
+enum pbr_kernel {
+  PBR_KERNEL_AUTO = 0,     /* scene kernel when every scene has <= 64 triangle
+                              slots, else the chunked kernel               */
+  PBR_KERNEL_CHUNKED = 1,  /* any triangle count; region state in LDS      */
+  PBR_KERNEL_SCENE = 2     /* one wave per scene, <= 64 triangle slots     */
+};
+
 typedef struct pbr_render_opts {
-  int32_t region_rows;        /* output rows per work item (multiple of 8)  */
-  int32_t persistent;         /* 1: persistent (scene, region) work queue   */
+  int32_t region_rows;        /* chunked: output rows per work item (x8)    */
+  int32_t persistent;         /* chunked: persistent (scene, region) queue  */
   int32_t workgroups;         /* persistent launch size, 0 = 8 per CU       */
-  int32_t reserved0;
+  int32_t kernel;             /* enum pbr_kernel                            */
+  int32_t bin_width;          /* scene: 16, 32 or 64 pixel-wide bins        */
+  int32_t bins_per_item;      /* scene: bins one work item walks, 0 = auto  */
+  int32_t lane_pixels;        /* scene: 4 or 8 pixels per lane, 0 = auto    */
+  int32_t reserved;
 } pbr_render_opts;
+
+/* One pose component: value of row i = src ? src[i * stride] : value. */
+typedef struct pbr_pose_component {
+  const float* src;
+  int64_t stride;             /* in floats                                  */
+  float value;
+  int32_t reserved;
+} pbr_pose_component;
+
+/* Instance poses -> model matrices M = [[R(h,p,r) * s, (x,y,z)], [0,0,0,1]]
+ * with R = Rz(h) Ry(p) Rx(r), angles in radians (shader_context.py:48-84). */
+typedef struct pbr_pose_desc {
+  float* transforms;          /* [rows][4][4] written                       */
+  float* rot3;                /* optional [rows][3][3] written: R           */
+  float* scale;               /* optional [rows] written: s                 */
+  int32_t rows;
+  int32_t reserved;
+  pbr_pose_component x, y, z, h, p, r, s;
+} pbr_pose_desc;
 
 /* Render every scene of `scene` into `out` on `stream`.
  * Replaces: renderer.py:377-389 (step: taskMgr.step + grab_pixels +
@@ -120,6 +150,15 @@ typedef struct pbr_render_opts {
 int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out,
                const pbr_render_opts* opts, void* workspace,
                size_t workspace_bytes, void* stream);
+
+/* Write `rows` instance matrices from per-component pose sources on
+ * `stream`.  Replaces the host path of node.py:116-154 (set_positions,
+ * set_hprs, set_scales, _upload_current_transforms) and the CartPole state
+ * mapping of envs/cartpole/renderer.py:98-138 (one launch per node). */
+int pbr_set_pose(const pbr_pose_desc* pose, void* stream);
+
+/* pbr_set_pose for `count` (<= 8) nodes in one launch. */
+int pbr_set_poses(const pbr_pose_desc* poses, int32_t count, void* stream);
 
 /* Device workspace pbr_render needs for this batch and launch shape.
  * No counterpart in the reference (GL owned its buffers). */

@@ -75,12 +75,47 @@ class pbr_frame_out(C.Structure):
     ]
 
 
+PBR_KERNEL_AUTO = 0
+PBR_KERNEL_CHUNKED = 1
+PBR_KERNEL_SCENE = 2
+
+
 class pbr_render_opts(C.Structure):
     _fields_ = [
         ("region_rows", C.c_int32),
         ("persistent", C.c_int32),
         ("workgroups", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("kernel", C.c_int32),
+        ("bin_width", C.c_int32),
+        ("bins_per_item", C.c_int32),
+        ("lane_pixels", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class pbr_pose_component(C.Structure):
+    _fields_ = [
+        ("src", C.c_void_p),
+        ("stride", C.c_int64),
+        ("value", C.c_float),
+        ("reserved", C.c_int32),
+    ]
+
+
+class pbr_pose_desc(C.Structure):
+    _fields_ = [
+        ("transforms", C.c_void_p),
+        ("rot3", C.c_void_p),
+        ("scale", C.c_void_p),
+        ("rows", C.c_int32),
+        ("reserved", C.c_int32),
+        ("x", pbr_pose_component),
+        ("y", pbr_pose_component),
+        ("z", pbr_pose_component),
+        ("h", pbr_pose_component),
+        ("p", pbr_pose_component),
+        ("r", pbr_pose_component),
+        ("s", pbr_pose_component),
     ]
 
 
@@ -100,12 +135,20 @@ def expected_layout() -> list[int]:
         C.sizeof(pbr_frame_out),
         pbr_frame_out.channels.offset,
         C.sizeof(pbr_render_opts),
+        pbr_render_opts.kernel.offset,
+        pbr_render_opts.bins_per_item.offset,
+        C.sizeof(pbr_pose_component),
+        C.sizeof(pbr_pose_desc),
+        pbr_pose_desc.x.offset,
+        pbr_pose_desc.s.offset,
     ]
 
 
 # Exported symbols of libpbr_hip.so, as declared in include/pbr_abi.h.
 EXPORTS = (
     "pbr_render",
+    "pbr_set_pose",
+    "pbr_set_poses",
     "pbr_workspace_bytes",
     "pbr_last_error",
     "pbr_abi_version",
@@ -124,6 +167,10 @@ def declare(lib: C.CDLL) -> C.CDLL:
         C.c_void_p,
     ]
     lib.pbr_render.restype = C.c_int
+    lib.pbr_set_pose.argtypes = [C.POINTER(pbr_pose_desc), C.c_void_p]
+    lib.pbr_set_pose.restype = C.c_int
+    lib.pbr_set_poses.argtypes = [C.POINTER(pbr_pose_desc), C.c_int32, C.c_void_p]
+    lib.pbr_set_poses.restype = C.c_int
     lib.pbr_workspace_bytes.argtypes = [
         C.POINTER(pbr_scene_desc),
         C.POINTER(pbr_render_opts),

@@ -49,3 +49,16 @@ def workspace_bytes(desc: _abi.pbr_scene_desc, opts: _abi.pbr_render_opts) -> in
     if rc != _abi.PBR_OK:
         raise RuntimeError(f"pbr_workspace_bytes failed (status {rc}): {last_error()}")
     return int(n.value)
+
+
+def set_pose(desc: _abi.pbr_pose_desc, stream_ptr: int) -> None:
+    rc = library().pbr_set_pose(ctypes.byref(desc), ctypes.c_void_p(stream_ptr or None))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_set_pose failed (status {rc}): {last_error()}")
+
+
+def set_poses(descs: list, stream_ptr: int) -> None:
+    arr = (_abi.pbr_pose_desc * len(descs))(*descs)
+    rc = library().pbr_set_poses(arr, len(descs), ctypes.c_void_p(stream_ptr or None))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_set_poses failed (status {rc}): {last_error()}")

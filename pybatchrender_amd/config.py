@@ -9,8 +9,13 @@ written for the reference still construct; they have no effect here.
 New fields (defaults keep the reference's behaviour):
     output_dtype   'uint8' (reference), 'bfloat16' or 'float32' colour output
     output_depth   also produce a [N, H, W] float32 window-depth tensor
-    region_rows    output rows per kernel work item (0 = automatic)
-    persistent     launch the persistent (scene, region) work-queue kernel
+    region_rows    chunked kernel: output rows per work item (0 = automatic)
+    persistent     chunked kernel over a persistent (scene, region) work queue
+    kernel         'auto', 'chunked' (any triangle count) or 'scene' (one wave
+                   per scene, at most 64 triangle slots per scene)
+    bin_width      scene kernel: 16, 32 or 64 pixel-wide bins (0 = automatic)
+    bins_per_item  scene kernel: bins walked by one work item (0 = automatic)
+    lane_pixels    scene kernel: 4 or 8 consecutive pixels per lane (0 = automatic)
 """
 from __future__ import annotations
 
@@ -74,6 +79,10 @@ class PBRConfig:
     output_depth: bool = False
     region_rows: int = 0
     persistent: bool = False
+    kernel: str = "auto"
+    bin_width: int = 0
+    bins_per_item: int = 0
+    lane_pixels: int = 0
 
     def process_resolution(self) -> None:
         """Atlas tiling rules of config.py:61-126 (cols = ceil(sqrt(N)))."""
@@ -163,6 +172,12 @@ class PBRConfig:
             raise ValueError("num_channels must be in [1, 4]")
         if self.region_rows < 0 or self.region_rows % 8:
             raise ValueError("region_rows must be 0 or a positive multiple of 8")
+        if self.kernel not in ("auto", "chunked", "scene"):
+            raise ValueError(f"Invalid kernel: {self.kernel}")
+        if self.bin_width not in (0, 16, 32, 64):
+            raise ValueError("bin_width must be 0, 16, 32 or 64")
+        if self.lane_pixels not in (0, 4, 8):
+            raise ValueError("lane_pixels must be 0, 4 or 8")
 
     def __post_init__(self) -> None:
         self.process_resolution()

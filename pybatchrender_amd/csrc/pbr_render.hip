@@ -1,21 +1,31 @@
 // pbr_render.hip - gfx950 batch rasteriser behind the C ABI of include/pbr_abi.h.
 //
-// One work item = (scene s, region r): a full-width strip of `region_rows`
-// output rows of scene s.  A 256-thread workgroup takes the item through
-//   (a) transform   per triangle slot: instance row M and scene VP are read
-//                   from the packed [rows][4][4] buffers (basic.vert:29-43);
+// Stages of one work item (the names follow the reference's GL pipeline):
+//   (a) transform   triangle slot -> instance row M and scene VP read from the
+//                   packed [rows][4][4] buffers, clip = VP*(M*v)
+//                   (basic.vert:29-43), world normal normalize(mat3(M)*n) (:53);
 //   (b) setup       near/far/guard-band clip, 1/256-pixel snap, back-face
 //                   cull, exact edge functions with a top-left bias, depth
 //                   plane, flat colour or smooth-normal block -> LDS records;
-//   (c) binning     each wave walks its 8x8 bins; lane r tests record r's
-//                   bounding box, a wave ballot compacts the overlapping
-//                   records and the wave visits them in record order;
-//   (d) depth       lane = pixel; (depth bits, triangle id) minimum kept in
-//                   LDS per pixel (the visibility buffer of the region);
-//   (e) resolve     the region's colour/depth/visibility are streamed to the
-//                   caller's NCHW tensors with 32-bit coalesced stores.
-// Triangle slots are processed in chunks of CHUNK; pixel state persists in
-// LDS across chunks, so any triangle count works in bounded LDS.
+//   (c) binning     per screen bin, lane r tests record r (bounding box, then
+//                   the exact edge values at the bin's extreme pixel centres:
+//                   reject / fully covered / partial); a wave ballot compacts
+//                   the survivors, visited in record order;
+//   (d) depth       per pixel minimum of (depth, triangle id) -> the
+//                   visibility of the bin;
+//   (e) resolve     Lambert + ambient colour of the winner (basic.frag:31-38)
+//                   written straight into the caller's NCHW tensors.
+//
+// Two kernels share (a), (b) and the arithmetic:
+//   scene_kernel    one wave per scene (or per band of bins of a scene) for
+//                   scenes of at most 64 triangle slots: all records stay in
+//                   LDS, bins are walked in turn, each lane keeps the state of
+//                   PX consecutive pixels in registers and stores them with
+//                   one vector store per channel.
+//   chunked_kernel  any triangle count: a 256-thread workgroup owns a strip of
+//                   rows whose per-pixel state lives in LDS across chunks of
+//                   64 triangle slots; optionally persistent over a device
+//                   work queue of (scene, strip) items.
 //
 // Arithmetic contract (shared with the CPU oracle, bit for bit): float ops in
 // the order written (built with -ffp-contract=off), IEEE division and sqrt,
@@ -23,8 +33,8 @@
 // evaluated exactly in fp64 (|value| < 2^51).
 
 #include <hip/hip_runtime.h>
-#include <stdint.h>
 #include <stdarg.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -32,23 +42,16 @@
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;  // chunked_kernel workgroup
 constexpr int kWaves = kBlock / 64;
+constexpr int kChunk = 64;   // triangle slots per setup round (chunked_kernel)
 constexpr int kMaxPoly = 9;
+constexpr int kSceneSlots = 64;  // scene_kernel: slots per scene and records per pass
+constexpr uint32_t kOne = 0x3f800000u;
 
 struct CVert {
   float p[4];  // clip position
   float n[3];  // normalised world normal
-};
-
-struct RasterRec {  // 88 bytes, read uniformly by a whole wave
-  double A[3], B[3], C[3];  // edge k: e = A*Px + B*Py + C (C carries the top-left bias)
-  float zx, zy, zc;         // depth plane in pixel units
-  uint32_t id;              // scene-local triangle id
-};
-
-struct BoxRec {  // GL pixel bounding box, already clipped to the region
-  int16_t i0, i1, j0, j1;
 };
 
 struct Light {
@@ -91,11 +94,34 @@ __device__ __forceinline__ uint32_t unorm8(float c) {
   return (uint32_t)floorf(c * 255.0f + 0.5f);
 }
 
+__device__ __forceinline__ uint32_t pack_rgba8(const float c[4]) {
+  return unorm8(c[0]) | (unorm8(c[1]) << 8) | (unorm8(c[2]) << 16) | (unorm8(c[3]) << 24);
+}
+
 __device__ __forceinline__ uint32_t bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
   if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
   u += 0x7fffu + ((u >> 16) & 1u);
   return u >> 16;
+}
+
+// Depth clamped to [0, 1] with -0 folded to +0 (the contract's clamp).
+__device__ __forceinline__ float clamp_depth(float z) {
+  z = fmaxf(z, 0.0f);
+  z = fminf(z, 1.0f);
+  return z + 0.0f;
+}
+
+__device__ __forceinline__ Light make_light(const pbr_scene_desc& D) {
+  Light lt;
+  normalize3(D.light_dir, lt.dir);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    lt.col[c] = D.light_col[c];
+    lt.amb[c] = D.ambient[c];
+  }
+  lt.s = fminf(fmaxf(D.strength, 0.0f), 1.0f);
+  return lt;
 }
 
 __device__ __forceinline__ float plane_dist(const CVert& v, int plane, float g) {
@@ -110,7 +136,7 @@ __device__ __forceinline__ float plane_dist(const CVert& v, int plane, float g) 
 }
 
 // Sutherland-Hodgman against near, far and the guard band.  Only reached
-// by triangles that straddle a plane; the private array lives in scratch.
+// by triangles that straddle a plane; the private arrays live in scratch.
 __device__ __noinline__ int clip_polygon(CVert* poly, float g) {
   CVert tmp[kMaxPoly];
   int n = 3;
@@ -141,43 +167,122 @@ __device__ __noinline__ int clip_polygon(CVert* poly, float g) {
   return n;
 }
 
-// Per-launch constants derived from the descriptor.
-struct Frame {
-  int W, H, rows_per_region, num_regions;
-  float guard;
+// all_in: no plane clips the triangle; all_out: one plane rejects it whole.
+__device__ __forceinline__ void clip_class(const CVert cv[3], float g, bool& all_in,
+                                           bool& all_out) {
+  all_in = true;
+  all_out = false;
+#pragma unroll
+  for (int pl = 0; pl < 6; ++pl) {
+    float d0 = plane_dist(cv[0], pl, g), d1 = plane_dist(cv[1], pl, g),
+          d2 = plane_dist(cv[2], pl, g);
+    all_in = all_in && d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f;
+    all_out = all_out || (d0 < 0.0f && d1 < 0.0f && d2 < 0.0f);
+  }
+}
+
+// Exclusive prefix of per-node triangle counts for scene s (node.py:45-46:
+// cnt instances of the node's mesh, counts[] clamped to [0, ips]).
+__device__ void node_table(const pbr_scene_desc& D, int s, uint32_t* base) {
+  uint32_t acc = 0;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    const pbr_node& nd = D.nodes[j];
+    const int ips = nd.instances_per_scene;
+    int cnt = ips;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), ips);
+    base[j] = acc;
+    acc += (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
+  }
+  base[D.num_nodes] = acc;
+}
+
+// (a) slot -> clip-space triangle of scene s, its instance colour and whether
+// the node is faceted.  Slot order = node, instance, triangle (draw order).
+__device__ __forceinline__ void fetch_node_triangle(const pbr_scene_desc& D, int s, int j,
+                                                    uint32_t local, CVert cv[3], float rgba[4],
+                                                    bool& flat) {
+  const pbr_node& nd = D.nodes[j];
+  const pbr_mesh& me = D.meshes[nd.mesh];
+  const uint32_t ntri = (uint32_t)me.num_triangles;
+  const uint32_t i = local / ntri, t = local - i * ntri;
+  const size_t row =
+      nd.shared ? (size_t)i : (size_t)s * (size_t)nd.instances_per_scene + (size_t)i;
+  const float4* M4 = reinterpret_cast<const float4*>(nd.transforms + row * 16);
+  const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2], m3 = M4[3];
+  const float M[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
+                       m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+  const float4 c4 = reinterpret_cast<const float4*>(nd.colors)[row];
+  rgba[0] = c4.x;
+  rgba[1] = c4.y;
+  rgba[2] = c4.z;
+  rgba[3] = c4.w;
+  flat = nd.flat != 0;
+  const float4* V4 = reinterpret_cast<const float4*>(D.view_proj + (size_t)s * 16);
+  const float4 q0 = V4[0], q1 = V4[1], q2 = V4[2], q3 = V4[3];
+  const float VP[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                        q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const int32_t* tri = D.indices + (size_t)(me.index_offset + (int)t) * 3;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float* vv = D.vertices + (size_t)(me.vertex_offset + tri[k]) * 6;
+    float wp[4], nw[3];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      float a = M[rr * 4 + 0] * vv[0] + M[rr * 4 + 1] * vv[1];
+      a = a + M[rr * 4 + 2] * vv[2];
+      wp[rr] = a + M[rr * 4 + 3];
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      float a = VP[rr * 4 + 0] * wp[0] + VP[rr * 4 + 1] * wp[1];
+      a = a + VP[rr * 4 + 2] * wp[2];
+      cv[k].p[rr] = a + VP[rr * 4 + 3] * wp[3];
+    }
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      float a = M[rr * 4 + 0] * vv[3] + M[rr * 4 + 1] * vv[4];
+      nw[rr] = a + M[rr * 4 + 2] * vv[5];
+    }
+    normalize3(nw, cv[k].n);
+  }
+}
+
+__device__ __forceinline__ void fetch_triangle(const pbr_scene_desc& D, int s,
+                                               const uint32_t* base, uint32_t slot,
+                                               CVert cv[3], float rgba[4], bool& flat) {
+  int j = 0;
+  while (slot >= base[j + 1]) ++j;
+  fetch_node_triangle(D, s, j, slot - base[j], cv, rgba, flat);
+}
+
+// Snapped triangle: the fixed-point facts every later stage derives from.
+struct FixTri {
+  int32_t X[3], Y[3];  // window position in 1/256 pixel, GL y up
+  float zx, zy, zc;    // depth plane in pixel units
+  int i0, i1, j0, j1;  // pixel bounding box clipped to the tile rows [jr0, jr1]
+  uint32_t bias;       // bit k: edge k is not top-left (strict inside test)
+  float invw[3];
 };
 
-// Triangle record in registers before it is committed to LDS.
-struct SubTri {
-  double A[3], B[3], C[3];
-  float zx, zy, zc;
-  int i0, i1, j0, j1;
-  uint32_t bias;
-  float sh[16];  // flat: shaded rgba; smooth: n0 n1 n2 (9), invw (3), rgba (4)
-};
-
-// (b) triangle setup of one clip-space sub-triangle; false if it produces no
-// fragment inside the region rows [jr0, jr1] (GL row numbers).
-template <bool SMOOTH>
-__device__ __forceinline__ bool setup_sub(const CVert& v0, const CVert& v1, const CVert& v2,
-                                          const Frame& F, int jr0, int jr1, bool flat,
-                                          const float flat_rgba[4], const float rgba[4],
-                                          SubTri& T) {
+// (b) snap, cull, bound, fill-rule bias and depth plane of one clip-space
+// triangle; false if it yields no fragment in GL rows [jr0, jr1].
+__device__ __forceinline__ bool setup_fixed(const CVert& v0, const CVert& v1, const CVert& v2,
+                                            int W, int H, int jr0, int jr1, FixTri& T) {
   const CVert* v[3] = {&v0, &v1, &v2};
   int64_t X[3], Y[3];
-  float d[3], invw[3];
+  float d[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     float w = v[k]->p[3];
     if (!(w > 0.0f)) return false;
     float nx = v[k]->p[0] / w, ny = v[k]->p[1] / w, nz = v[k]->p[2] / w;
-    float sx = (nx + 1.0f) * (0.5f * (float)F.W);
-    float sy = (ny + 1.0f) * (0.5f * (float)F.H);
+    float sx = (nx + 1.0f) * (0.5f * (float)W);
+    float sy = (ny + 1.0f) * (0.5f * (float)H);
     if (!(fabsf(sx) < 65536.0f) || !(fabsf(sy) < 65536.0f)) return false;  // non-finite input
     X[k] = (int64_t)rintf(sx * 256.0f);
     Y[k] = (int64_t)rintf(sy * 256.0f);
     d[k] = nz * 0.5f + 0.5f;
-    invw[k] = 1.0f / w;
+    T.invw[k] = 1.0f / w;
   }
   int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
   if (area <= 0) return false;
@@ -186,7 +291,7 @@ __device__ __forceinline__ bool setup_sub(const CVert& v0, const CVert& v1, cons
   int64_t i0 = (xmin - 128 + 255) >> 8, i1 = (xmax - 128) >> 8;
   int64_t j0 = (ymin - 128 + 255) >> 8, j1 = (ymax - 128) >> 8;
   i0 = max(i0, (int64_t)0);
-  i1 = min(i1, (int64_t)(F.W - 1));
+  i1 = min(i1, (int64_t)(W - 1));
   j0 = max(j0, (int64_t)jr0);
   j1 = min(j1, (int64_t)jr1);
   if (i0 > i1 || j0 > j1) return false;
@@ -198,18 +303,11 @@ __device__ __forceinline__ bool setup_sub(const CVert& v0, const CVert& v1, cons
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int a = (k + 1) % 3, b = (k + 2) % 3;
-    int64_t A = Y[a] - Y[b];
-    int64_t B = X[b] - X[a];
-    int64_t C = -(A * X[a] + B * Y[a]);
-    int64_t dx = X[b] - X[a], dy = Y[b] - Y[a];
-    bool tl = (dy < 0) || (dy == 0 && dx < 0);
-    if (!tl) {
-      C -= 1;
-      bias |= 1u << k;
-    }
-    T.A[k] = (double)A;
-    T.B[k] = (double)B;
-    T.C[k] = (double)C;
+    const int64_t dx = X[b] - X[a], dy = Y[b] - Y[a];
+    const bool tl = (dy < 0) || (dy == 0 && dx < 0);
+    if (!tl) bias |= 1u << k;
+    T.X[k] = (int32_t)X[k];
+    T.Y[k] = (int32_t)Y[k];
   }
   T.bias = bias;
   double x0 = (double)X[0] * (1.0 / 256.0), y0 = (double)Y[0] * (1.0 / 256.0);
@@ -224,15 +322,63 @@ __device__ __forceinline__ bool setup_sub(const CVert& v0, const CVert& v1, cons
   T.zx = (float)zxd;
   T.zy = (float)zyd;
   T.zc = (float)zcd;
+  return true;
+}
+
+// Exact edge functions e_k = A*Px + B*Py + C of a snapped triangle (edge k is
+// opposite vertex k); C carries the fill-rule bias so inside <=> e >= 0.
+__device__ __forceinline__ void edges_from_fixed(const int32_t X[3], const int32_t Y[3],
+                                                 uint32_t bias, double A[3], double B[3],
+                                                 double C[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int a = (k + 1) % 3, b = (k + 2) % 3;
+    const int64_t Ak = (int64_t)Y[a] - (int64_t)Y[b];
+    const int64_t Bk = (int64_t)X[b] - (int64_t)X[a];
+    const int64_t Ck = -(Ak * (int64_t)X[a] + Bk * (int64_t)Y[a]) - (int64_t)((bias >> k) & 1u);
+    A[k] = (double)Ak;
+    B[k] = (double)Bk;
+    C[k] = (double)Ck;
+  }
+}
+
+// Triangle record in registers before it is committed to LDS.
+struct SubTri {
+  double A[3], B[3], C[3];
+  float zx, zy, zc;
+  int i0, i1, j0, j1;
+  uint32_t bias;
+  float sh[16];  // flat: shaded rgba; smooth: n0 n1 n2 (9), invw (3), rgba (4)
+};
+
+// (b) triangle setup of one clip-space sub-triangle; false if it produces no
+// fragment inside GL rows [jr0, jr1] of a W x H tile.
+template <bool SMOOTH>
+__device__ __forceinline__ bool setup_sub(const CVert& v0, const CVert& v1, const CVert& v2,
+                                          int W, int H, int jr0, int jr1, bool flat,
+                                          const float flat_rgba[4], const float rgba[4],
+                                          SubTri& T) {
+  FixTri F;
+  if (!setup_fixed(v0, v1, v2, W, H, jr0, jr1, F)) return false;
+  edges_from_fixed(F.X, F.Y, F.bias, T.A, T.B, T.C);
+  T.zx = F.zx;
+  T.zy = F.zy;
+  T.zc = F.zc;
+  T.i0 = F.i0;
+  T.i1 = F.i1;
+  T.j0 = F.j0;
+  T.j1 = F.j1;
+  T.bias = F.bias;
   if (!SMOOTH || flat) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) T.sh[c] = flat_rgba[c];
   } else {
+    const CVert* v[3] = {&v0, &v1, &v2};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) T.sh[k * 3 + c] = v[k]->n[c];
-      T.sh[9 + k] = invw[k];
+      T.sh[9 + k] = F.invw[k];
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) T.sh[12 + c] = rgba[c];
@@ -240,36 +386,97 @@ __device__ __forceinline__ bool setup_sub(const CVert& v0, const CVert& v1, cons
   return true;
 }
 
-// LDS layout (dynamic shared memory).  REC = records per round.
-template <int CHUNK, bool SMOOTH, int CMODE>
-struct Lds {
-  static constexpr int REC = CHUNK + CHUNK / 2;
+// Perspective-correct smooth shading of a record at pixel centre (px, py)
+// (subpixel units): weights = unbiased edge value / w of the opposite vertex.
+__device__ __forceinline__ void shade_smooth(const double A[3], const double B[3],
+                                             const double C[3], uint32_t fl, const float* sh,
+                                             double px, double py, const Light& lt,
+                                             float col[4]) {
+  float bw[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double e = fma(A[k], px, fma(B[k], py, C[k]));
+    e = e + (double)((fl >> k) & 1u);
+    bw[k] = (float)e * sh[9 + k];
+  }
+  float n[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float a = bw[0] * sh[c] + bw[1] * sh[3 + c];
+    n[c] = a + bw[2] * sh[6 + c];
+  }
+  const float rgba[4] = {sh[12], sh[13], sh[14], sh[15]};
+  shade(n, rgba, lt, col);
+}
+
+// ---------------------------------------------------------------------------
+// chunked_kernel: any triangle count, per-pixel state in LDS.
+// ---------------------------------------------------------------------------
+struct RasterRec {  // 88 bytes, read uniformly by a whole wave
+  double A[3], B[3], C[3];  // edge k: e = A*Px + B*Py + C (C carries the top-left bias)
+  float zx, zy, zc;         // depth plane in pixel units
+  uint32_t id;              // scene-local triangle id
+};
+
+struct BoxRec {  // GL pixel bounding box, already clipped to the region
+  int16_t i0, i1, j0, j1;
+};
+
+struct Frame {
+  int W, H, rows_per_region, num_regions;
+  float guard;
+};
+
+template <bool SMOOTH, int CMODE>
+struct ChunkLds {
+  static constexpr int REC = kChunk + kChunk / 2;
   static constexpr int SHW = SMOOTH ? 16 : 4;  // shade floats per record
   static constexpr size_t off_rec = 0;
   static constexpr size_t off_box = off_rec + sizeof(RasterRec) * REC;
   static constexpr size_t off_flags = off_box + sizeof(BoxRec) * REC;
   static constexpr size_t off_shade = off_flags + 4 * REC;
   static constexpr size_t off_misc = off_shade + 4 * SHW * REC;
-  static constexpr size_t misc_words = 2 * (PBR_MAX_NODES + 1) + 4;
+  static constexpr size_t misc_words = (PBR_MAX_NODES + 1) + 4;
   static constexpr size_t off_px = (off_misc + 4 * misc_words + 15) & ~(size_t)15;
   static constexpr size_t px_bytes = 8 + (CMODE == 0 ? 4 : 16);
   static size_t bytes(int px) { return off_px + px_bytes * (size_t)px; }
 };
 
-template <int CHUNK, bool SMOOTH, int CMODE>
-__global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_frame_out O,
-                                                        Frame F, uint32_t* queue,
-                                                        uint32_t total_items) {
-  using L = Lds<CHUNK, SMOOTH, CMODE>;
+template <bool SMOOTH, int CMODE>
+__device__ __forceinline__ void chunk_write(const SubTri& T, uint32_t rec, uint32_t slot, bool flat,
+                                            RasterRec* s_rec, BoxRec* s_box, uint32_t* s_flags,
+                                            float* s_shade) {
+  using L = ChunkLds<SMOOTH, CMODE>;
+  RasterRec& R = s_rec[rec];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    R.A[k] = T.A[k];
+    R.B[k] = T.B[k];
+    R.C[k] = T.C[k];
+  }
+  R.zx = T.zx;
+  R.zy = T.zy;
+  R.zc = T.zc;
+  R.id = slot;
+  s_box[rec] = BoxRec{(int16_t)T.i0, (int16_t)T.i1, (int16_t)T.j0, (int16_t)T.j1};
+  s_flags[rec] = T.bias | ((SMOOTH && !flat) ? 8u : 0u);
+#pragma unroll
+  for (int c = 0; c < L::SHW; ++c) s_shade[rec * L::SHW + c] = T.sh[c];
+}
+
+template <bool SMOOTH, int CMODE>
+__global__ __launch_bounds__(kBlock) void chunked_kernel(pbr_scene_desc D, pbr_frame_out O,
+                                                         Frame F, uint32_t* queue,
+                                                         uint32_t total_items) {
+  using L = ChunkLds<SMOOTH, CMODE>;
   extern __shared__ __align__(16) unsigned char lds[];
   RasterRec* s_rec = reinterpret_cast<RasterRec*>(lds + L::off_rec);
   BoxRec* s_box = reinterpret_cast<BoxRec*>(lds + L::off_box);
   uint32_t* s_flags = reinterpret_cast<uint32_t*>(lds + L::off_flags);
   float* s_shade = reinterpret_cast<float*>(lds + L::off_shade);
   uint32_t* s_misc = reinterpret_cast<uint32_t*>(lds + L::off_misc);
-  uint32_t* s_node_base = s_misc;                        // [PBR_MAX_NODES + 1]
-  uint32_t* s_node_cnt = s_misc + (PBR_MAX_NODES + 1);   // [PBR_MAX_NODES + 1]
-  uint32_t* s_nrec = s_misc + 2 * (PBR_MAX_NODES + 1);
+  uint32_t* s_node_base = s_misc;  // [PBR_MAX_NODES + 1]
+  uint32_t* s_nrec = s_misc + (PBR_MAX_NODES + 1);
   uint32_t* s_item = s_nrec + 1;
   const int rpx = F.rows_per_region * F.W;
   uint32_t* s_depth = reinterpret_cast<uint32_t*>(lds + L::off_px);
@@ -280,16 +487,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_fr
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int W = F.W, H = F.H;
-
-  Light lt;
-  normalize3(D.light_dir, lt.dir);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    lt.col[c] = D.light_col[c];
-    lt.amb[c] = D.ambient[c];
-  }
-  lt.s = fminf(fmaxf(D.strength, 0.0f), 1.0f);
-  const uint32_t kOne = 0x3f800000u;
+  const Light lt = make_light(D);
 
   uint32_t item = blockIdx.x;
   if (queue) {
@@ -307,11 +505,8 @@ __global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_fr
     const int jr0 = H - y1, jr1 = H - 1 - y0;  // GL rows covered by the region
 
     // pixel state: clear depth 1.0, id 0, background colour
-    uint32_t bg_packed = 0;
-    float bgc[4] = {D.background[0], D.background[1], D.background[2], D.background[3]};
-    if (CMODE == 0)
-      bg_packed = unorm8(bgc[0]) | (unorm8(bgc[1]) << 8) | (unorm8(bgc[2]) << 16) |
-                  (unorm8(bgc[3]) << 24);
+    const float bgc[4] = {D.background[0], D.background[1], D.background[2], D.background[3]};
+    const uint32_t bg_packed = pack_rgba8(bgc);
     for (int p = tid; p < npx; p += kBlock) {
       s_depth[p] = kOne;
       s_id[p] = 0;
@@ -320,105 +515,34 @@ __global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_fr
       else
         reinterpret_cast<float4*>(s_col)[p] = make_float4(bgc[0], bgc[1], bgc[2], bgc[3]);
     }
-    if (tid == 0) {
-      uint32_t acc = 0;
-      for (int j = 0; j < D.num_nodes; ++j) {
-        const pbr_node& nd = D.nodes[j];
-        int ips = nd.instances_per_scene;
-        int cnt = ips;
-        if (nd.counts) cnt = min(max(nd.counts[s], 0), ips);
-        uint32_t tris = (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
-        s_node_base[j] = acc;
-        s_node_cnt[j] = tris;
-        acc += tris;
-      }
-      s_node_base[D.num_nodes] = acc;
-    }
+    if (tid == 0) node_table(D, s, s_node_base);
     __syncthreads();
     const uint32_t total = s_node_base[D.num_nodes];
-    const float* VP = D.view_proj + (size_t)s * 16;
 
-    for (uint32_t base = 0; base < total; base += CHUNK) {
+    for (uint32_t base = 0; base < total; base += kChunk) {
       const uint32_t slot = base + (uint32_t)tid;
-      bool pending = tid < CHUNK && slot < total;
+      bool pending = tid < kChunk && slot < total;
       int resume = 0;  // sub-triangles of this slot already handled
       while (true) {
         if (tid == 0) *s_nrec = 0;
         __syncthreads();
         if (pending) {
-          // locate node j, instance i, triangle t of this slot
-          int j = 0;
-          while (slot >= s_node_base[j + 1]) ++j;
-          const pbr_node& nd = D.nodes[j];
-          const pbr_mesh& me = D.meshes[nd.mesh];
-          const uint32_t local = slot - s_node_base[j];
-          const uint32_t ntri = (uint32_t)me.num_triangles;
-          const uint32_t i = local / ntri, t = local % ntri;
-          const size_t row = nd.shared ? (size_t)i
-                                       : (size_t)s * (size_t)nd.instances_per_scene + (size_t)i;
-          const float* M = nd.transforms + row * 16;
-          float rgba[4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) rgba[c] = nd.colors[row * 4 + c];
-          const int32_t* tri = D.indices + (size_t)(me.index_offset + (int)t) * 3;
           CVert cv[3];
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            const float* vv = D.vertices + (size_t)(me.vertex_offset + tri[k]) * 6;
-            float wp[4], nw[3];
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-              float a = M[rr * 4 + 0] * vv[0] + M[rr * 4 + 1] * vv[1];
-              a = a + M[rr * 4 + 2] * vv[2];
-              wp[rr] = a + M[rr * 4 + 3];
-            }
-#pragma unroll
-            for (int rr = 0; rr < 4; ++rr) {
-              float a = VP[rr * 4 + 0] * wp[0] + VP[rr * 4 + 1] * wp[1];
-              a = a + VP[rr * 4 + 2] * wp[2];
-              cv[k].p[rr] = a + VP[rr * 4 + 3] * wp[3];
-            }
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-              float a = M[rr * 4 + 0] * vv[3] + M[rr * 4 + 1] * vv[4];
-              nw[rr] = a + M[rr * 4 + 2] * vv[5];
-            }
-            normalize3(nw, cv[k].n);
-          }
-          const bool flat = nd.flat != 0;
+          float rgba[4];
+          bool flat;
+          fetch_triangle(D, s, s_node_base, slot, cv, rgba, flat);
           float flat_rgba[4];
           if (!SMOOTH || flat) shade(cv[0].n, rgba, lt, flat_rgba);
-          // fast path: every vertex inside every plane -> no clipping
-          bool all_in = true, all_out = false;
-#pragma unroll
-          for (int pl = 0; pl < 6; ++pl) {
-            float d0 = plane_dist(cv[0], pl, F.guard), d1 = plane_dist(cv[1], pl, F.guard),
-                  d2 = plane_dist(cv[2], pl, F.guard);
-            all_in = all_in && d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f;
-            all_out = all_out || (d0 < 0.0f && d1 < 0.0f && d2 < 0.0f);
-          }
+          bool all_in, all_out;
+          clip_class(cv, F.guard, all_in, all_out);
           if (all_out) {
             pending = false;
           } else if (all_in) {
             SubTri T;
-            if (setup_sub<SMOOTH>(cv[0], cv[1], cv[2], F, jr0, jr1, flat, flat_rgba, rgba, T)) {
+            if (setup_sub<SMOOTH>(cv[0], cv[1], cv[2], W, H, jr0, jr1, flat, flat_rgba, rgba, T)) {
               uint32_t rec = atomicAdd(s_nrec, 1u);
               if (rec < (uint32_t)L::REC) {
-                RasterRec& R = s_rec[rec];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                  R.A[k] = T.A[k];
-                  R.B[k] = T.B[k];
-                  R.C[k] = T.C[k];
-                }
-                R.zx = T.zx;
-                R.zy = T.zy;
-                R.zc = T.zc;
-                R.id = slot;
-                s_box[rec] = BoxRec{(int16_t)T.i0, (int16_t)T.i1, (int16_t)T.j0, (int16_t)T.j1};
-                s_flags[rec] = T.bias | ((SMOOTH && !flat) ? 8u : 0u);
-#pragma unroll
-                for (int c = 0; c < L::SHW; ++c) s_shade[rec * L::SHW + c] = T.sh[c];
+                chunk_write<SMOOTH, CMODE>(T, rec, slot, flat, s_rec, s_box, s_flags, s_shade);
                 pending = false;
               }
             } else {
@@ -433,24 +557,12 @@ __global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_fr
             int k = resume + 1;
             for (; k + 1 < n; ++k) {
               SubTri T;
-              if (!setup_sub<SMOOTH>(poly[0], poly[k], poly[k + 1], F, jr0, jr1, flat, flat_rgba,
-                                     rgba, T))
+              if (!setup_sub<SMOOTH>(poly[0], poly[k], poly[k + 1], W, H, jr0, jr1, flat,
+                                     flat_rgba, rgba, T))
                 continue;
               uint32_t rec = atomicAdd(s_nrec, 1u);
               if (rec >= (uint32_t)L::REC) break;
-              RasterRec& R = s_rec[rec];
-              for (int q = 0; q < 3; ++q) {
-                R.A[q] = T.A[q];
-                R.B[q] = T.B[q];
-                R.C[q] = T.C[q];
-              }
-              R.zx = T.zx;
-              R.zy = T.zy;
-              R.zc = T.zc;
-              R.id = slot;
-              s_box[rec] = BoxRec{(int16_t)T.i0, (int16_t)T.i1, (int16_t)T.j0, (int16_t)T.j1};
-              s_flags[rec] = T.bias | ((SMOOTH && !flat) ? 8u : 0u);
-              for (int c = 0; c < L::SHW; ++c) s_shade[rec * L::SHW + c] = T.sh[c];
+              chunk_write<SMOOTH, CMODE>(T, rec, slot, flat, s_rec, s_box, s_flags, s_shade);
             }
             resume = k - 1;
             if (k + 1 >= n) pending = false;
@@ -491,10 +603,7 @@ __global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_fr
               const double e1 = fma(R.A[1], px, fma(R.B[1], py, R.C[1]));
               const double e2 = fma(R.A[2], px, fma(R.B[2], py, R.C[2]));
               if (e0 >= 0.0 && e1 >= 0.0 && e2 >= 0.0) {
-                float z = fmaf(R.zx, pxf, fmaf(R.zy, pyf, R.zc));
-                z = fmaxf(z, 0.0f);
-                z = fminf(z, 1.0f);
-                z = z + 0.0f;
+                const float z = clamp_depth(fmaf(R.zx, pxf, fmaf(R.zy, pyf, R.zc)));
                 const uint32_t zb = __float_as_uint(z);
                 if (zb < bd || (zb == bd && R.id < bid)) {
                   bd = zb;
@@ -512,29 +621,13 @@ __global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_fr
             const float* sh = s_shade + (size_t)best * L::SHW;
             if (SMOOTH && (fl & 8u)) {
               const RasterRec& R = s_rec[best];
-              float bw[3];
-#pragma unroll
-              for (int k = 0; k < 3; ++k) {
-                double e = fma(R.A[k], px, fma(R.B[k], py, R.C[k]));
-                e = e + (double)((fl >> k) & 1u);
-                bw[k] = (float)e * sh[9 + k];
-              }
-              float n[3];
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                float a = bw[0] * sh[c] + bw[1] * sh[3 + c];
-                n[c] = a + bw[2] * sh[6 + c];
-              }
-              float rgba[4] = {sh[12], sh[13], sh[14], sh[15]};
-              shade(n, rgba, lt, col);
+              shade_smooth(R.A, R.B, R.C, fl, sh, px, py, lt, col);
             } else {
 #pragma unroll
               for (int c = 0; c < 4; ++c) col[c] = sh[c];
             }
             if (CMODE == 0)
-              reinterpret_cast<uint32_t*>(s_col)[p] =
-                  unorm8(col[0]) | (unorm8(col[1]) << 8) | (unorm8(col[2]) << 16) |
-                  (unorm8(col[3]) << 24);
+              reinterpret_cast<uint32_t*>(s_col)[p] = pack_rgba8(col);
             else
               reinterpret_cast<float4*>(s_col)[p] = make_float4(col[0], col[1], col[2], col[3]);
           }
@@ -597,10 +690,513 @@ __global__ __launch_bounds__(kBlock) void render_kernel(pbr_scene_desc D, pbr_fr
   }
 }
 
+// ---------------------------------------------------------------------------
+// small-scene path: setup_kernel (a)+(b) -> records in the workspace ->
+// raster_kernel (c)-(e), for scenes of at most 64 triangle slots.
+// ---------------------------------------------------------------------------
+
+// Snapped triangle record passed from setup_kernel to raster_kernel (64 B).
+struct GRec {
+  int32_t X[3], Y[3];      // window position, 1/256 pixel, GL y up
+  float zx, zy, zc;        // depth plane in pixel units
+  uint32_t id;             // scene-local triangle id
+  int16_t i0, i1, j0, j1;  // pixel bounding box; i0 = INT16_MAX when culled
+  uint32_t flags;          // bits 0-2 fill-rule bias of edges 0-2, bit 3 smooth
+  uint32_t color;          // packed RGBA8 of a flat record
+  uint32_t pad[2];
+};
+static_assert(sizeof(GRec) == 64, "GRec is one 64-byte line");
+
+struct SmallFrame {
+  int W, H;
+  float guard;
+  int seg;              // setup lanes per scene (power of two >= slots)
+  int cap;              // records per scene in the workspace
+  int nbx, nby;         // bins per row / column
+  int bins_per_item;    // consecutive bins one raster work item walks
+  int items_per_scene;
+};
+
+// Shade floats kept per record beside the GRec array: flat rgba, or for
+// smooth records n0 n1 n2 (9), 1/w (3), rgba (4).
+constexpr int kShadeFloats = 16;
+
+template <bool SMOOTH>
+__device__ __forceinline__ void write_grec(const FixTri& T, uint32_t id, bool flat,
+                                           const float flat_rgba[4], const float rgba[4],
+                                           const CVert& v0, const CVert& v1, const CVert& v2,
+                                           GRec* g, float* sh) {
+  GRec r;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    r.X[k] = T.X[k];
+    r.Y[k] = T.Y[k];
+  }
+  r.zx = T.zx;
+  r.zy = T.zy;
+  r.zc = T.zc;
+  r.id = id;
+  r.i0 = (int16_t)T.i0;
+  r.i1 = (int16_t)T.i1;
+  r.j0 = (int16_t)T.j0;
+  r.j1 = (int16_t)T.j1;
+  const bool smooth = SMOOTH && !flat;
+  r.flags = T.bias | (smooth ? 8u : 0u);
+  r.color = smooth ? 0u : pack_rgba8(flat_rgba);
+  r.pad[0] = r.pad[1] = 0;
+  const uint4* src = reinterpret_cast<const uint4*>(&r);
+  uint4* dst = reinterpret_cast<uint4*>(g);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = src[q];
+  float4* s4 = reinterpret_cast<float4*>(sh);
+  if (!smooth) {
+    s4[0] = make_float4(flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]);
+  } else {
+    s4[0] = make_float4(v0.n[0], v0.n[1], v0.n[2], v1.n[0]);
+    s4[1] = make_float4(v1.n[1], v1.n[2], v2.n[0], v2.n[1]);
+    s4[2] = make_float4(v2.n[2], T.invw[0], T.invw[1], T.invw[2]);
+    s4[3] = make_float4(rgba[0], rgba[1], rgba[2], rgba[3]);
+  }
+}
+
+__device__ __forceinline__ void write_dead(uint32_t id, GRec* g) {
+  GRec r = {};
+  r.id = id;
+  r.i0 = INT16_MAX;
+  r.i1 = -1;
+  r.j0 = INT16_MAX;
+  r.j1 = -1;
+  const uint4* src = reinterpret_cast<const uint4*>(&r);
+  uint4* dst = reinterpret_cast<uint4*>(g);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = src[q];
+}
+
+// (a)+(b): lane = (scene, triangle slot); a scene owns `seg` consecutive
+// lanes, so the record numbering is a segmented prefix sum in slot order.
+template <bool SMOOTH>
+__global__ __launch_bounds__(256) void setup_kernel(pbr_scene_desc D, SmallFrame F, GRec* recs,
+                                                    float* shades, uint32_t* counts) {
+  const int gtid = (int)(blockIdx.x * 256 + threadIdx.x);
+  const int lane = (int)(threadIdx.x & 63);
+  const int s = gtid / F.seg;
+  const uint32_t slot = (uint32_t)(gtid % F.seg);
+  const bool scene_ok = s < D.num_scenes;
+  const Light lt = make_light(D);
+
+  // node of this slot: walk the per-node triangle counts of scene s
+  int node = -1;
+  uint32_t local = 0, total = 0;
+  if (scene_ok) {
+    for (int j = 0; j < D.num_nodes; ++j) {
+      const pbr_node& nd = D.nodes[j];
+      const int ips = nd.instances_per_scene;
+      int cnt = ips;
+      if (nd.counts) cnt = min(max(nd.counts[s], 0), ips);
+      const uint32_t n = (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
+      if (node < 0 && slot < total + n) {
+        node = j;
+        local = slot - total;
+      }
+      total += n;
+    }
+  }
+  CVert cv[3];
+  float rgba[4];
+  bool flat = true, all_in = false, all_out = true;
+  int cnt = 0;
+  CVert poly[kMaxPoly];
+  if (node >= 0) {
+    fetch_node_triangle(D, s, node, local, cv, rgba, flat);
+    clip_class(cv, F.guard, all_in, all_out);
+    if (all_out) {
+      cnt = 0;
+    } else if (all_in) {
+      cnt = 1;
+    } else {
+      poly[0] = cv[0];
+      poly[1] = cv[1];
+      poly[2] = cv[2];
+      cnt = max(clip_polygon(poly, F.guard) - 2, 0);
+    }
+  }
+  int incl = cnt;
+  for (int d = 1; d < F.seg; d <<= 1) {
+    const int t = __shfl_up(incl, d, F.seg);
+    if ((int)slot >= d) incl += t;
+  }
+  const int seg_last = (lane & ~(F.seg - 1)) + F.seg - 1;
+  const int nrec = __shfl(incl, seg_last, 64);
+  if (!scene_ok) return;
+  if (slot == 0) counts[s] = (uint32_t)nrec;
+  if (cnt == 0) return;
+  float flat_rgba[4];
+  if (!SMOOTH || flat) shade(cv[0].n, rgba, lt, flat_rgba);
+  const int pos = incl - cnt;
+  GRec* g = recs + (size_t)s * F.cap;
+  float* sh = shades + (size_t)s * F.cap * kShadeFloats;
+  for (int q = 0; q < cnt; ++q) {
+    const int r = pos + q;
+    FixTri T;
+    bool ok;
+    if (all_in)
+      ok = setup_fixed(cv[0], cv[1], cv[2], F.W, F.H, 0, F.H - 1, T);
+    else
+      ok = setup_fixed(poly[0], poly[q + 1], poly[q + 2], F.W, F.H, 0, F.H - 1, T);
+    if (!ok) {
+      write_dead(slot, g + r);
+    } else if (all_in) {
+      write_grec<SMOOTH>(T, slot, flat, flat_rgba, rgba, cv[0], cv[1], cv[2], g + r,
+                         sh + (size_t)r * kShadeFloats);
+    } else {
+      write_grec<SMOOTH>(T, slot, flat, flat_rgba, rgba, poly[0], poly[q + 1], poly[q + 2],
+                         g + r, sh + (size_t)r * kShadeFloats);
+    }
+  }
+}
+
+struct SceneRec {  // 96 bytes in LDS
+  double A[3], B[3], C[3];
+  float zx, zy, zc;
+  uint32_t id;
+  int16_t i0, i1, j0, j1;
+};
+
+template <bool SMOOTH, int CMODE>
+struct RasterLds {
+  static constexpr int SHW = SMOOTH ? 16 : (CMODE == 0 ? 1 : 4);
+  SceneRec rec[64];
+  float shade[64 * SHW];
+  uint32_t flags[64];
+};
+
+// Records [64*pass, 64*pass + nl) of scene s -> LDS (lane r takes record r).
+template <bool SMOOTH, int CMODE>
+__device__ __forceinline__ void load_pass(const GRec* recs, const float* shade, int pass, int nl,
+                                          RasterLds<SMOOTH, CMODE>& L) {
+  constexpr int SHW = RasterLds<SMOOTH, CMODE>::SHW;
+  const int r = (int)threadIdx.x;
+  if (r >= nl) return;
+  const int gi = 64 * pass + r;
+  const uint4* src = reinterpret_cast<const uint4*>(recs + gi);
+  GRec g;
+  uint4* dst = reinterpret_cast<uint4*>(&g);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = src[q];
+  SceneRec& R = L.rec[r];
+  edges_from_fixed(g.X, g.Y, g.flags & 7u, R.A, R.B, R.C);
+  R.zx = g.zx;
+  R.zy = g.zy;
+  R.zc = g.zc;
+  R.id = g.id;
+  R.i0 = g.i0;
+  R.i1 = g.i1;
+  R.j0 = g.j0;
+  R.j1 = g.j1;
+  L.flags[r] = g.flags;
+  if (SHW == 1) {
+    reinterpret_cast<uint32_t*>(L.shade)[r] = g.color;
+  } else {
+    const float4* s4 = reinterpret_cast<const float4*>(shade + (size_t)gi * kShadeFloats);
+    float4* d4 = reinterpret_cast<float4*>(&L.shade[r * SHW]);
+#pragma unroll
+    for (int q = 0; q < SHW / 4; ++q) d4[q] = s4[q];
+  }
+}
+
+// (c)-(e): one wave per (scene, band of bins).  PX consecutive pixels per
+// lane, LPR lanes per bin row: bins of (PX*LPR) x (64/LPR) pixels.
+template <int PX, int LPR, bool SMOOTH, int CMODE>
+__global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame F,
+                                                    const GRec* recs, const float* shade,
+                                                    const uint32_t* counts, Light lt,
+                                                    float4 bg) {
+  constexpr int BW = PX * LPR;
+  constexpr int BH = 64 / LPR;
+  using LT = RasterLds<SMOOTH, CMODE>;
+  constexpr int SHW = LT::SHW;
+  __shared__ LT L;
+  const int lane = threadIdx.x;
+  const int s = (int)(blockIdx.x / (uint32_t)F.items_per_scene);
+  const int part = (int)(blockIdx.x % (uint32_t)F.items_per_scene);
+  const int nb = F.nbx * F.nby;
+  const int b_begin = part * F.bins_per_item;
+  const int b_end = min(nb, b_begin + F.bins_per_item);
+  const int W = F.W, H = F.H;
+  const float bgc[4] = {bg.x, bg.y, bg.z, bg.w};
+  const GRec* grec = recs + (size_t)s * F.cap;
+  const float* gsh = shade + (size_t)s * F.cap * kShadeFloats;
+  const int nrec = (int)min(counts[s], (uint32_t)F.cap);
+  const int npass = max(1, (nrec + 63) / 64);
+  load_pass<SMOOTH, CMODE>(grec, gsh, 0, min(64, nrec), L);
+  __syncthreads();
+  int loaded = 0;
+
+  const int lx = lane % LPR, ly = lane / LPR;
+  const int C = O.channels;
+  const size_t plane = (size_t)H * (size_t)W;
+
+  for (int b = b_begin; b < b_end; ++b) {
+    const int bxi = b % F.nbx, byi = b / F.nbx;
+    const int x0 = bxi * BW + lx * PX;
+    const int yo = byi * BH + ly;
+    const int jg = H - 1 - yo;
+    const double px0 = (double)(256 * x0 + 128), py = (double)(256 * jg + 128);
+    const float pyf = (float)jg + 0.5f;
+    float pxf[PX];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) pxf[i] = (float)(x0 + i) + 0.5f;
+    // bin rectangle of pixel centres (GL rows)
+    const int bi0 = bxi * BW, bi1 = bi0 + BW - 1;
+    const int bj1 = H - 1 - byi * BH, bj0 = bj1 - (BH - 1);
+    const double cx0 = (double)(256 * bi0 + 128), cx1 = (double)(256 * bi1 + 128);
+    const double cy0 = (double)(256 * bj0 + 128), cy1 = (double)(256 * bj1 + 128);
+
+    float bd[PX];
+    uint32_t bid[PX];
+    uint32_t colp[PX];
+    float4 colf[PX];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+      bd[i] = 1.0f;
+      bid[i] = 0;
+      if (CMODE == 0)
+        colp[i] = pack_rgba8(bgc);
+      else
+        colf[i] = bg;
+    }
+
+    for (int pass = 0; pass < npass; ++pass) {
+      if (pass != loaded) {  // more than 64 records: reload this pass
+        __syncthreads();
+        load_pass<SMOOTH, CMODE>(grec, gsh, pass, min(64, nrec - 64 * pass), L);
+        __syncthreads();
+        loaded = pass;
+      }
+      const int nl = min(64, nrec - 64 * pass);
+      int brec[PX];
+#pragma unroll
+      for (int i = 0; i < PX; ++i) brec[i] = -1;
+
+      // (c) classify record `lane` against this bin: box, then the exact
+      // edge values at the bin's extreme pixel centres
+      int cls = 0;
+      if (lane < nl) {
+        const SceneRec& R = L.rec[lane];
+        const bool ov = (R.i0 <= bi1) & (R.i1 >= bi0) & (R.j0 <= bj1) & (R.j1 >= bj0);
+        if (ov) {
+          bool full = true, out = false;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const bool ap = R.A[k] >= 0.0, bp = R.B[k] >= 0.0;
+            const double xa = ap ? cx0 : cx1, xb = ap ? cx1 : cx0;
+            const double ya = bp ? cy0 : cy1, yb = bp ? cy1 : cy0;
+            const double emin = fma(R.A[k], xa, fma(R.B[k], ya, R.C[k]));
+            const double emax = fma(R.A[k], xb, fma(R.B[k], yb, R.C[k]));
+            full = full & (emin >= 0.0);
+            out = out | (emax < 0.0);
+          }
+          cls = out ? 0 : (full ? 2 : 1);
+        }
+      }
+      uint64_t m_any = __ballot(cls != 0);
+      const uint64_t m_full = __ballot(cls == 2);
+
+      // (d) depth: records in id order, so a strictly smaller depth wins
+      while (m_any) {
+        const int k = __ffsll((unsigned long long)m_any) - 1;
+        m_any &= m_any - 1;
+        const SceneRec& R = L.rec[k];
+        const float t = fmaf(R.zy, pyf, R.zc);
+        if ((m_full >> k) & 1ull) {
+#pragma unroll
+          for (int i = 0; i < PX; ++i) {
+            const float z = clamp_depth(fmaf(R.zx, pxf[i], t));
+            if (z < bd[i]) {
+              bd[i] = z;
+              brec[i] = k;
+            }
+          }
+        } else {
+          double e0 = fma(R.A[0], px0, fma(R.B[0], py, R.C[0]));
+          double e1 = fma(R.A[1], px0, fma(R.B[1], py, R.C[1]));
+          double e2 = fma(R.A[2], px0, fma(R.B[2], py, R.C[2]));
+          const double d0 = R.A[0] * 256.0, d1 = R.A[1] * 256.0, d2 = R.A[2] * 256.0;
+#pragma unroll
+          for (int i = 0; i < PX; ++i) {
+            const float z = clamp_depth(fmaf(R.zx, pxf[i], t));
+            if ((e0 >= 0.0) & (e1 >= 0.0) & (e2 >= 0.0) & (z < bd[i])) {
+              bd[i] = z;
+              brec[i] = k;
+            }
+            e0 += d0;
+            e1 += d1;
+            e2 += d2;
+          }
+        }
+      }
+
+      // colour of pixels whose winner changed in this pass
+#pragma unroll
+      for (int i = 0; i < PX; ++i) {
+        if (brec[i] >= 0) {
+          const int r = brec[i];
+          bid[i] = L.rec[r].id;
+          const uint32_t fl = L.flags[r];
+          if (SMOOTH && (fl & 8u)) {
+            const SceneRec& R = L.rec[r];
+            float col[4];
+            shade_smooth(R.A, R.B, R.C, fl, &L.shade[r * SHW], (double)(256 * (x0 + i) + 128),
+                         py, lt, col);
+            if (CMODE == 0)
+              colp[i] = pack_rgba8(col);
+            else
+              colf[i] = make_float4(col[0], col[1], col[2], col[3]);
+          } else if (SHW == 1) {
+            colp[i] = reinterpret_cast<const uint32_t*>(L.shade)[r];
+          } else {
+            const float* sh = &L.shade[r * SHW];
+            if (CMODE == 0) {
+              const float c4[4] = {sh[0], sh[1], sh[2], sh[3]};
+              colp[i] = pack_rgba8(c4);
+            } else {
+              colf[i] = make_float4(sh[0], sh[1], sh[2], sh[3]);
+            }
+          }
+        }
+      }
+    }
+
+    // (e) resolve this bin (lanes past the image edge store nothing; with
+    // W % PX == 0 a lane is either wholly inside or wholly outside)
+    if (yo < H && x0 < W) {
+      const size_t rowoff = (size_t)yo * (size_t)W + (size_t)x0;
+      const bool vec = (W % PX) == 0;
+      for (int c = 0; c < C; ++c) {
+        const size_t off = ((size_t)s * C + c) * plane + rowoff;
+        if (O.dtype == PBR_DTYPE_U8) {
+          uint8_t* dst = reinterpret_cast<uint8_t*>(O.color) + off;
+          uint32_t v[PX];
+#pragma unroll
+          for (int i = 0; i < PX; ++i) v[i] = (colp[i] >> (8 * c)) & 255u;
+          if (vec) {
+#pragma unroll
+            for (int i = 0; i < PX; i += 4)
+              reinterpret_cast<uint32_t*>(dst)[i / 4] =
+                  v[i] | (v[i + 1] << 8) | (v[i + 2] << 16) | (v[i + 3] << 24);
+          } else {
+#pragma unroll
+            for (int i = 0; i < PX; ++i)
+              if (x0 + i < W) dst[i] = (uint8_t)v[i];
+          }
+        } else {
+          float v[PX];
+#pragma unroll
+          for (int i = 0; i < PX; ++i)
+            v[i] = c == 0 ? colf[i].x : (c == 1 ? colf[i].y : (c == 2 ? colf[i].z : colf[i].w));
+          if (O.dtype == PBR_DTYPE_F32) {
+            float* dst = reinterpret_cast<float*>(O.color) + off;
+            if (vec) {
+#pragma unroll
+              for (int i = 0; i < PX; i += 4)
+                *reinterpret_cast<float4*>(dst + i) = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+            } else {
+#pragma unroll
+              for (int i = 0; i < PX; ++i)
+                if (x0 + i < W) dst[i] = v[i];
+            }
+          } else {
+            uint16_t* dst = reinterpret_cast<uint16_t*>(O.color) + off;
+            if (vec) {
+#pragma unroll
+              for (int i = 0; i < PX; i += 4)
+                *reinterpret_cast<uint2*>(dst + i) =
+                    make_uint2(bf16_rne(v[i]) | (bf16_rne(v[i + 1]) << 16),
+                               bf16_rne(v[i + 2]) | (bf16_rne(v[i + 3]) << 16));
+            } else {
+#pragma unroll
+              for (int i = 0; i < PX; ++i)
+                if (x0 + i < W) dst[i] = (uint16_t)bf16_rne(v[i]);
+            }
+          }
+        }
+      }
+      const size_t doff = (size_t)s * plane + rowoff;
+      if (O.depth) {
+        float* dst = O.depth + doff;
+        if (vec) {
+#pragma unroll
+          for (int i = 0; i < PX; i += 4)
+            *reinterpret_cast<float4*>(dst + i) = make_float4(bd[i], bd[i + 1], bd[i + 2], bd[i + 3]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < PX; ++i)
+            if (x0 + i < W) dst[i] = bd[i];
+        }
+      }
+      if (O.vis) {
+        int32_t* dst = O.vis + doff;
+        int32_t v[PX];
+#pragma unroll
+        for (int i = 0; i < PX; ++i) v[i] = bd[i] < 1.0f ? (int32_t)bid[i] : -1;
+        if (vec) {
+#pragma unroll
+          for (int i = 0; i < PX; i += 4)
+            *reinterpret_cast<int4*>(dst + i) = make_int4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < PX; ++i)
+            if (x0 + i < W) dst[i] = v[i];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pose_kernel: instance poses -> model matrices (one thread per row).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pose_value(const pbr_pose_component& c, int64_t i) {
+  return c.src ? c.src[i * c.stride] : c.value;
+}
+
+constexpr int kMaxPoseBatch = 8;
+struct PoseBatch {
+  pbr_pose_desc d[kMaxPoseBatch];
+};
+
+// blockIdx.y selects the descriptor, so one launch poses several nodes.
+__global__ __launch_bounds__(256) void pose_kernel(PoseBatch B) {
+  const pbr_pose_desc& P = B.d[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.rows) return;
+  const float x = pose_value(P.x, i), y = pose_value(P.y, i), z = pose_value(P.z, i);
+  const float h = pose_value(P.h, i), p = pose_value(P.p, i), r = pose_value(P.r, i);
+  const float sc = pose_value(P.s, i);
+  const float ch = cosf(h), sh = sinf(h), cp = cosf(p), sp = sinf(p), cr = cosf(r), sr = sinf(r);
+  const float chsp = ch * sp, shsp = sh * sp;
+  const float R[9] = {ch * cp, chsp * sr - sh * cr, chsp * cr + sh * sr,
+                      sh * cp, shsp * sr + ch * cr, shsp * cr - ch * sr,
+                      -sp,     cp * sr,             cp * cr};
+  float4* M = reinterpret_cast<float4*>(P.transforms + i * 16);
+  M[0] = make_float4(R[0] * sc, R[1] * sc, R[2] * sc, x);
+  M[1] = make_float4(R[3] * sc, R[4] * sc, R[5] * sc, y);
+  M[2] = make_float4(R[6] * sc, R[7] * sc, R[8] * sc, z);
+  M[3] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (P.rot3) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P.rot3[i * 9 + k] = R[k];
+  }
+  if (P.scale) P.scale[i] = sc;
+}
+
 // Resets the persistent queue head inside the stream order (graph-safe: a
 // kernel, not a memset node).
 __global__ void reset_queue_kernel(uint32_t* q) { *q = 0u; }
 
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
 thread_local char g_err[512] = "";
 
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -644,76 +1240,249 @@ int validate(const pbr_scene_desc* D, const pbr_frame_out* O) {
       if (!nd.transforms || !nd.colors)
         return fail(PBR_ERR_INVALID, "node %d: null transforms/colors", j);
       if (!D->vertices || !D->indices) return fail(PBR_ERR_INVALID, "null vertex/index pool");
+      if (((uintptr_t)nd.transforms & 15) || ((uintptr_t)nd.colors & 15))
+        return fail(PBR_ERR_INVALID, "node %d: transforms/colors not 16-byte aligned", j);
     }
     tris_per_scene += (uint64_t)nd.instances_per_scene * (uint64_t)me.num_triangles;
   }
+  if (D->num_scenes > 0 && ((uintptr_t)D->view_proj & 15))
+    return fail(PBR_ERR_INVALID, "view_proj not 16-byte aligned");
   if (tris_per_scene >= 0x7fffffffull)
     return fail(PBR_ERR_INVALID, "%llu triangle slots per scene exceed 2^31",
                 (unsigned long long)tris_per_scene);
   return PBR_OK;
 }
 
-struct Launch {
-  Frame F;
-  int chunk;
-  bool smooth;
-  int cmode;
-  size_t lds;
-  uint32_t items;
-};
-
-template <int CHUNK, bool SMOOTH, int CMODE>
-size_t lds_bytes_for(int px) {
-  return Lds<CHUNK, SMOOTH, CMODE>::bytes(px);
-}
-
-size_t lds_bytes(const Launch& l) {
-  const int px = l.F.rows_per_region * l.F.W;
-  if (l.smooth) return l.cmode == 0 ? lds_bytes_for<64, true, 0>(px) : lds_bytes_for<64, true, 1>(px);
-  return l.cmode == 0 ? lds_bytes_for<64, false, 0>(px) : lds_bytes_for<64, false, 1>(px);
-}
-
-int plan(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts, Launch* l) {
-  l->F.W = D->width;
-  l->F.H = D->height;
-  l->F.guard = 65536.0f / (float)(D->width > D->height ? D->width : D->height);
-  l->chunk = 64;
-  l->smooth = false;
+uint64_t max_slots(const pbr_scene_desc* D) {
+  uint64_t t = 0;
   for (int j = 0; j < D->num_nodes; ++j)
-    if (!D->nodes[j].flat) l->smooth = true;
-  l->cmode = O->dtype == PBR_DTYPE_U8 ? 0 : 1;
+    t += (uint64_t)D->nodes[j].instances_per_scene *
+         (uint64_t)D->meshes[D->nodes[j].mesh].num_triangles;
+  return t;
+}
+
+bool any_smooth(const pbr_scene_desc* D) {
+  for (int j = 0; j < D->num_nodes; ++j)
+    if (!D->nodes[j].flat) return true;
+  return false;
+}
+
+int device_cus() {
+  static thread_local int cached_dev = -1, cached_cus = 0;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev == cached_dev) return cached_cus;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 256;
+  cached_dev = dev;
+  cached_cus = cus > 0 ? cus : 256;
+  return cached_cus;
+}
+
+// --- chunked kernel launch ---------------------------------------------------
+template <bool SMOOTH, int CMODE>
+hipError_t launch_chunked_t(const pbr_scene_desc* D, const pbr_frame_out* O, const Frame& F,
+                            size_t lds, uint32_t items, uint32_t* queue, int grid,
+                            hipStream_t st) {
+  auto k = chunked_kernel<SMOOTH, CMODE>;
+  if (lds > 64 * 1024) {
+    hipError_t e =
+        hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), lds, st, *D, *O, F, queue, items);
+  return hipGetLastError();
+}
+
+int launch_chunked(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
+                   void* workspace, size_t workspace_bytes, hipStream_t st) {
+  Frame F;
+  F.W = D->width;
+  F.H = D->height;
+  F.guard = 65536.0f / (float)(D->width > D->height ? D->width : D->height);
+  const bool smooth = any_smooth(D);
+  const int cmode = O->dtype == PBR_DTYPE_U8 ? 0 : 1;
   int rows = opts ? opts->region_rows : 0;
   if (rows <= 0) {
-    // about 1024 pixels per work item, whole 8-row bins
-    rows = (1024 / D->width) & ~7;
+    rows = (1024 / D->width) & ~7;  // about 1024 pixels per work item
     if (rows < 8) rows = 8;
   }
   if (rows % 8) return fail(PBR_ERR_INVALID, "region_rows %d is not a multiple of 8", rows);
   const int hpad = (D->height + 7) & ~7;
   if (rows > hpad) rows = hpad;
-  l->F.rows_per_region = rows;
-  l->F.num_regions = (D->height + rows - 1) / rows;
-  l->lds = lds_bytes(*l);
-  if (l->lds > 160 * 1024)
-    return fail(PBR_ERR_UNSUPPORTED, "work item needs %zu bytes of LDS (width %d, rows %d)",
-                l->lds, D->width, rows);
-  const uint64_t items = (uint64_t)D->num_scenes * (uint64_t)l->F.num_regions;
-  if (items >= 0xffffffffull) return fail(PBR_ERR_INVALID, "too many work items");
-  l->items = (uint32_t)items;
+  F.rows_per_region = rows;
+  F.num_regions = (D->height + rows - 1) / rows;
+  const int px = rows * D->width;
+  size_t lds;
+  if (smooth)
+    lds = cmode == 0 ? ChunkLds<true, 0>::bytes(px) : ChunkLds<true, 1>::bytes(px);
+  else
+    lds = cmode == 0 ? ChunkLds<false, 0>::bytes(px) : ChunkLds<false, 1>::bytes(px);
+  if (lds > 160 * 1024)
+    return fail(PBR_ERR_UNSUPPORTED, "work item needs %zu bytes of LDS (width %d, rows %d)", lds,
+                D->width, rows);
+  const uint64_t items64 = (uint64_t)D->num_scenes * (uint64_t)F.num_regions;
+  if (items64 >= 0x7fffffffull) return fail(PBR_ERR_INVALID, "too many work items");
+  const uint32_t items = (uint32_t)items64;
+  uint32_t* queue = nullptr;
+  int grid = (int)items;
+  if (opts && opts->persistent) {
+    if (!workspace || workspace_bytes < 256)
+      return fail(PBR_ERR_WORKSPACE, "persistent launch needs a 256-byte workspace");
+    queue = reinterpret_cast<uint32_t*>(workspace);
+    grid = opts->workgroups > 0 ? opts->workgroups : 8 * device_cus();
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, st, queue);
+  }
+  hipError_t e;
+  if (smooth)
+    e = cmode == 0 ? launch_chunked_t<true, 0>(D, O, F, lds, items, queue, grid, st)
+                   : launch_chunked_t<true, 1>(D, O, F, lds, items, queue, grid, st);
+  else
+    e = cmode == 0 ? launch_chunked_t<false, 0>(D, O, F, lds, items, queue, grid, st)
+                   : launch_chunked_t<false, 1>(D, O, F, lds, items, queue, grid, st);
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "launch failed: %s", hipGetErrorString(e));
   return PBR_OK;
 }
 
-template <int CHUNK, bool SMOOTH, int CMODE>
-hipError_t launch_t(const pbr_scene_desc* D, const pbr_frame_out* O, const Launch& l,
-                    uint32_t* queue, int grid, hipStream_t st) {
-  auto k = render_kernel<CHUNK, SMOOTH, CMODE>;
-  if (l.lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)l.lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), l.lds, st, *D, *O, l.F, queue, l.items);
+// --- small-scene path launch -------------------------------------------------
+// Workspace of the small path: [GRec cap*N][shade floats cap*N*16][counts N].
+struct SmallPlan {
+  int slots, seg, cap;
+  size_t off_shade, off_counts, bytes;
+};
+
+SmallPlan small_plan(const pbr_scene_desc* D) {
+  SmallPlan p;
+  p.slots = (int)max_slots(D);
+  p.seg = 1;
+  while (p.seg < p.slots) p.seg <<= 1;
+  // worst case 7 records per slot (a triangle clipped by six planes)
+  p.cap = 7 * (p.slots > 0 ? p.slots : 1);
+  const size_t n = (size_t)(D->num_scenes > 0 ? D->num_scenes : 0);
+  const size_t recs = n * (size_t)p.cap * sizeof(GRec);
+  p.off_shade = (recs + 255) & ~(size_t)255;
+  const size_t shades = n * (size_t)p.cap * kShadeFloats * sizeof(float);
+  p.off_counts = (p.off_shade + shades + 255) & ~(size_t)255;
+  p.bytes = p.off_counts + n * sizeof(uint32_t);
+  return p;
+}
+
+bool use_small(const pbr_scene_desc* D, const pbr_render_opts* opts) {
+  const int kernel = opts ? opts->kernel : PBR_KERNEL_AUTO;
+  if (kernel == PBR_KERNEL_CHUNKED) return false;
+  if (kernel == PBR_KERNEL_SCENE) return true;
+  return max_slots(D) <= (uint64_t)kSceneSlots && !(opts && opts->persistent);
+}
+
+template <int PX, int LPR, bool SMOOTH>
+hipError_t launch_raster_t(const pbr_frame_out* O, const SmallFrame& F, int grid, const GRec* recs,
+                           const float* shade, const uint32_t* counts, const Light& lt,
+                           float4 bg, hipStream_t st) {
+  if (O->dtype == PBR_DTYPE_U8)
+    hipLaunchKernelGGL((raster_kernel<PX, LPR, SMOOTH, 0>), dim3(grid), dim3(64), 0, st, *O, F,
+                       recs, shade, counts, lt, bg);
+  else
+    hipLaunchKernelGGL((raster_kernel<PX, LPR, SMOOTH, 1>), dim3(grid), dim3(64), 0, st, *O, F,
+                       recs, shade, counts, lt, bg);
   return hipGetLastError();
+}
+
+template <int PX, int LPR>
+int launch_small_bins(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
+                      const SmallPlan& P, unsigned char* ws, hipStream_t st) {
+  constexpr int BW = PX * LPR, BH = 64 / LPR;
+  SmallFrame F;
+  F.W = D->width;
+  F.H = D->height;
+  F.guard = 65536.0f / (float)(D->width > D->height ? D->width : D->height);
+  F.seg = P.seg;
+  F.cap = P.cap;
+  F.nbx = (D->width + BW - 1) / BW;
+  F.nby = (D->height + BH - 1) / BH;
+  const int nb = F.nbx * F.nby;
+  int per = opts ? opts->bins_per_item : 0;
+  if (per <= 0) {
+    // enough raster work items to fill the chip: about 8 waves per CU
+    const int want = 8 * device_cus();
+    int items = (want + D->num_scenes - 1) / D->num_scenes;
+    if (items < 1) items = 1;
+    if (items > nb) items = nb;
+    per = (nb + items - 1) / items;
+  }
+  if (per > nb) per = nb;
+  F.bins_per_item = per;
+  F.items_per_scene = (nb + per - 1) / per;
+  const uint64_t grid64 = (uint64_t)D->num_scenes * (uint64_t)F.items_per_scene;
+  const uint64_t lanes64 = (uint64_t)D->num_scenes * (uint64_t)P.seg;
+  if (grid64 >= 0x7fffffffull || lanes64 >= 0x7fffffffull)
+    return fail(PBR_ERR_INVALID, "too many work items");
+  GRec* recs = reinterpret_cast<GRec*>(ws);
+  float* shade = reinterpret_cast<float*>(ws + P.off_shade);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(ws + P.off_counts);
+  const bool smooth = any_smooth(D);
+  const int sgrid = (int)((lanes64 + 255) / 256);
+  if (smooth)
+    hipLaunchKernelGGL(setup_kernel<true>, dim3(sgrid), dim3(256), 0, st, *D, F, recs, shade, counts);
+  else
+    hipLaunchKernelGGL(setup_kernel<false>, dim3(sgrid), dim3(256), 0, st, *D, F, recs, shade,
+                       counts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "setup launch failed: %s", hipGetErrorString(e));
+  // the light as the kernels see it (same arithmetic as make_light)
+  Light lt;
+  {
+    const float* v = D->light_dir;
+    float d = v[0] * v[0] + v[1] * v[1];
+    d = d + v[2] * v[2];
+    const float r = sqrtf(d);
+    for (int c = 0; c < 3; ++c) {
+      lt.dir[c] = r > 0.0f ? v[c] / r : 0.0f;
+      lt.col[c] = D->light_col[c];
+      lt.amb[c] = D->ambient[c];
+    }
+    lt.s = fminf(fmaxf(D->strength, 0.0f), 1.0f);
+  }
+  const float4 bg = make_float4(D->background[0], D->background[1], D->background[2],
+                                D->background[3]);
+  e = smooth ? launch_raster_t<PX, LPR, true>(O, F, (int)grid64, recs, shade, counts, lt, bg, st)
+             : launch_raster_t<PX, LPR, false>(O, F, (int)grid64, recs, shade, counts, lt, bg, st);
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "raster launch failed: %s", hipGetErrorString(e));
+  return PBR_OK;
+}
+
+int launch_small(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
+                 void* workspace, size_t workspace_bytes, hipStream_t st) {
+  if (max_slots(D) > (uint64_t)kSceneSlots)
+    return fail(PBR_ERR_UNSUPPORTED, "scene kernel needs at most %d triangle slots per scene",
+                kSceneSlots);
+  const SmallPlan P = small_plan(D);
+  if (!workspace || workspace_bytes < P.bytes)
+    return fail(PBR_ERR_WORKSPACE, "small-scene path needs a %zu-byte workspace (got %zu)", P.bytes,
+                workspace_bytes);
+  if ((uintptr_t)workspace & 255) return fail(PBR_ERR_WORKSPACE, "workspace not 256-byte aligned");
+  unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
+  // measured on MI355X for 64x64 tiles: 8 pixels per lane in 32 x 16 bins
+  int bw = opts ? opts->bin_width : 0;
+  if (bw <= 0) bw = D->width > 16 ? 32 : 16;
+  int px = opts ? opts->lane_pixels : 0;
+  if (px <= 0) px = 8;
+  if (px == 4) {
+    switch (bw) {
+      case 64: return launch_small_bins<4, 16>(D, O, opts, P, ws, st);
+      case 32: return launch_small_bins<4, 8>(D, O, opts, P, ws, st);
+      case 16: return launch_small_bins<4, 4>(D, O, opts, P, ws, st);
+      default: break;
+    }
+  } else if (px == 8) {
+    switch (bw) {
+      case 64: return launch_small_bins<8, 8>(D, O, opts, P, ws, st);
+      case 32: return launch_small_bins<8, 4>(D, O, opts, P, ws, st);
+      case 16: return launch_small_bins<8, 2>(D, O, opts, P, ws, st);
+      default: break;
+    }
+  }
+  return fail(PBR_ERR_INVALID, "bin_width %d / lane_pixels %d is not one of 16|32|64 x 4|8", bw,
+              px);
 }
 
 }  // namespace
@@ -739,17 +1508,60 @@ int pbr_abi_layout(int64_t* out, int32_t n) {
       (int64_t)sizeof(pbr_frame_out),
       (int64_t)offsetof(pbr_frame_out, channels),
       (int64_t)sizeof(pbr_render_opts),
+      (int64_t)offsetof(pbr_render_opts, kernel),
+      (int64_t)offsetof(pbr_render_opts, bins_per_item),
+      (int64_t)sizeof(pbr_pose_component),
+      (int64_t)sizeof(pbr_pose_desc),
+      (int64_t)offsetof(pbr_pose_desc, x),
+      (int64_t)offsetof(pbr_pose_desc, s),
   };
   const int32_t total = (int32_t)(sizeof(facts) / sizeof(facts[0]));
   for (int32_t i = 0; i < n && i < total; ++i) out[i] = facts[i];
   return total;
 }
 
+int pbr_set_poses(const pbr_pose_desc* poses, int32_t count, void* stream) {
+  if (count < 0 || count > kMaxPoseBatch)
+    return fail(PBR_ERR_INVALID, "pose count %d outside [0, %d]", count, kMaxPoseBatch);
+  if (count > 0 && !poses) return fail(PBR_ERR_INVALID, "null pose descriptors");
+  PoseBatch B;
+  memset(&B, 0, sizeof(B));
+  int32_t max_rows = 0;
+  for (int k = 0; k < count; ++k) {
+    const pbr_pose_desc& p = poses[k];
+    if (p.rows < 0) return fail(PBR_ERR_INVALID, "pose %d: rows < 0", k);
+    if (p.rows > 0 && (!p.transforms || ((uintptr_t)p.transforms & 15)))
+      return fail(PBR_ERR_INVALID, "pose %d: transforms must be a 16-byte aligned pointer", k);
+    B.d[k] = p;
+    if (p.rows > max_rows) max_rows = p.rows;
+  }
+  g_err[0] = 0;
+  if (max_rows == 0) return PBR_OK;
+  const int grid = (max_rows + 255) / 256;
+  hipLaunchKernelGGL(pose_kernel, dim3(grid, count), dim3(256), 0, (hipStream_t)stream, B);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "launch failed: %s", hipGetErrorString(e));
+  return PBR_OK;
+}
+
+int pbr_set_pose(const pbr_pose_desc* pose, void* stream) {
+  if (!pose) return fail(PBR_ERR_INVALID, "null pose descriptor");
+  return pbr_set_poses(pose, 1, stream);
+}
+
 int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts, size_t* bytes) {
   if (!bytes) return fail(PBR_ERR_INVALID, "null bytes");
-  (void)scene;
-  *bytes = (opts && opts->persistent) ? 256 : 0;
+  if (!scene) return fail(PBR_ERR_INVALID, "null descriptor");
+  if (scene->num_nodes < 0 || scene->num_nodes > PBR_MAX_NODES)
+    return fail(PBR_ERR_INVALID, "num_nodes %d outside [0, %d]", scene->num_nodes, PBR_MAX_NODES);
+  for (int j = 0; j < scene->num_nodes; ++j)
+    if (scene->nodes[j].mesh < 0 || scene->nodes[j].mesh >= PBR_MAX_MESHES)
+      return fail(PBR_ERR_INVALID, "node %d: mesh %d out of range", j, scene->nodes[j].mesh);
   g_err[0] = 0;
+  if (use_small(scene, opts))
+    *bytes = small_plan(scene).bytes;
+  else
+    *bytes = (opts && opts->persistent) ? 256 : 0;
   return PBR_OK;
 }
 
@@ -759,31 +1571,12 @@ int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out, const pbr_
   if (rc) return rc;
   g_err[0] = 0;
   if (scene->num_scenes == 0) return PBR_OK;
-  Launch l;
-  rc = plan(scene, out, opts, &l);
-  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  uint32_t* queue = nullptr;
-  int grid = (int)l.items;
-  if (opts && opts->persistent) {
-    if (!workspace || workspace_bytes < 256)
-      return fail(PBR_ERR_WORKSPACE, "persistent launch needs a 256-byte workspace");
-    queue = reinterpret_cast<uint32_t*>(workspace);
-    int dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    grid = opts->workgroups > 0 ? opts->workgroups : 8 * (cus > 0 ? cus : 256);
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, st, queue);
-  }
-  hipError_t e;
-  if (l.smooth)
-    e = l.cmode == 0 ? launch_t<64, true, 0>(scene, out, l, queue, grid, st)
-                     : launch_t<64, true, 1>(scene, out, l, queue, grid, st);
-  else
-    e = l.cmode == 0 ? launch_t<64, false, 0>(scene, out, l, queue, grid, st)
-                     : launch_t<64, false, 1>(scene, out, l, queue, grid, st);
-  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "launch failed: %s", hipGetErrorString(e));
-  return PBR_OK;
+  const int kernel = opts ? opts->kernel : PBR_KERNEL_AUTO;
+  if (kernel != PBR_KERNEL_AUTO && kernel != PBR_KERNEL_CHUNKED && kernel != PBR_KERNEL_SCENE)
+    return fail(PBR_ERR_INVALID, "unknown kernel %d", kernel);
+  if (use_small(scene, opts)) return launch_small(scene, out, opts, workspace, workspace_bytes, st);
+  return launch_chunked(scene, out, opts, workspace, workspace_bytes, st);
 }
 
 }  // extern "C"

@@ -90,8 +90,7 @@ class CartPoleRenderer(PBRRenderer):
         self.pole_hpr[:, :, 1] = math.pi * 0.5
         self.pole.set_hprs(self.pole_hpr)
 
-        self.cart_x_pos = self.cart_pos[:, :, 0:1].clone()
-        self.pole_theta = torch.zeros_like(self.cart_x_pos)
+        self.pole_y = (self.cart_size[1] + self.pole_size[1]) * 0.5
 
         self.add_camera()
         self._pbr_cam.set_positions(torch.tensor([5, 5, 2], dtype=torch.float32))
@@ -100,20 +99,16 @@ class CartPoleRenderer(PBRRenderer):
         self.setup_environment()
 
     def _step(self, state_batch: torch.Tensor | None = None):
-        """State -> cart position and pole position + pitch (renderer.py:98-138)."""
+        """State -> cart position and pole position + pitch (renderer.py:98-138):
+        cart M = T(x, 0, 0), pole M = T(x, 0.45, 0) Ry(theta), one pose launch
+        for both nodes reading the state columns in place."""
         if state_batch is None:
             return
         st = _state_rows(state_batch, int(self.cfg.num_scenes), self.device)
-        x = st[:, 0:1]
-        theta = st[:, 2:3]
-        self.cart_x_pos[:, :, 0] = x
-        self.cart_pos[:, :, 0:1] = self.cart_x_pos
-        self.cart.set_positions(self.cart_pos)
-        self.pole_pos[:, :, 0:1] = self.cart_x_pos
-        self.pole.set_positions(self.pole_pos, lazy=True)
-        self.pole_theta[:, :, 0] = theta
-        self.pole_hpr[:, :, 1:2] = self.pole_theta
-        self.pole.set_hprs(self.pole_hpr)
+        x = st[:, 0]
+        self.set_poses([(self.cart, dict(x=x, s=self.cart.scale_b11.reshape(-1))),
+                        (self.pole, dict(x=x, y=self.pole_y, p=st[:, 2],
+                                         s=self.pole.scale_b11.reshape(-1)))])
 
 
 class CartPoleGroundRenderer(PBRRenderer):
@@ -158,12 +153,11 @@ class CartPoleGroundRenderer(PBRRenderer):
         self.setup_environment()
 
     def _step(self, state_batch: torch.Tensor | None = None):
+        """State (x, x_dot, theta, theta_dot) -> cart T(x, 0, 0.1) and pole
+        T(x, 0, 0.2) Ry(theta): one pose launch for both nodes."""
         if state_batch is None:
             return
         st = _state_rows(state_batch, int(self.cfg.num_scenes), self.device)
-        self.cart_pos[:, 0] = st[:, 0]
-        self.pole_pos[:, 0] = st[:, 0]
-        self.pole_hpr[:, 1] = st[:, 2]
-        self.cart.set_positions(self.cart_pos)
-        self.pole.set_positions(self.pole_pos, lazy=True)
-        self.pole.set_hprs(self.pole_hpr)
+        x = st[:, 0]
+        self.set_poses([(self.cart, dict(x=x, z=0.5 * self.CART_SCALE[2])),
+                        (self.pole, dict(x=x, z=self.CART_SCALE[2], p=st[:, 2]))])

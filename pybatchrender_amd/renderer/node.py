@@ -157,6 +157,81 @@ class PBRNode(PBRShaderContext):
         self.rot3_b33 = self.rot3_b33 / self.scale_b11
         self._upload_current_transforms()
 
+    def set_pose(self, x=0.0, y=0.0, z=0.0, h=0.0, p=0.0, r=0.0, s=1.0) -> None:
+        """Set every buffer row's pose in one pass:
+        M = [[R(h, p, r) * s, (x, y, z)], [0, 0, 0, 1]], R = Rz(h) Ry(p) Rx(r).
+
+        The fused form of set_positions + set_hprs + set_scales + upload
+        (node.py:116-154).  Each argument is a number or a 1-D tensor with a
+        value per row (strided views such as ``state[:, 0]`` are read in
+        place).  On the GPU this is one HIP launch (pbr_set_pose);
+        PBRRenderer.set_poses poses several nodes in one launch."""
+        prep = self._pose_inputs(x=x, y=y, z=z, h=h, p=p, r=r, s=s)
+        if prep is None:
+            return
+        if self.device.type == "cuda":
+            from .. import _native
+
+            desc, keep = prep
+            _native.set_poses([desc], torch.cuda.current_stream(self.device).cuda_stream)
+            del keep
+            return
+        self._pose_host(prep)
+
+    def _pose_inputs(self, **comps):
+        """Output buffers sized for the pose, and the pose descriptor (GPU) or
+        the full per-row component tensors (CPU)."""
+        if not self.has_geometry:
+            return None
+        B = self.buf_instances
+        dev = self.device
+        if self.transforms_b44.shape != (B, 4, 4) or not self.transforms_b44.is_contiguous():
+            self.transforms_b44 = torch.eye(4, dtype=torch.float32, device=dev).repeat(B, 1, 1)
+        if self.rot3_b33.shape != (B, 3, 3) or not self.rot3_b33.is_contiguous():
+            self.rot3_b33 = torch.empty((B, 3, 3), dtype=torch.float32, device=dev)
+        if self.scale_b11.shape != (B, 1, 1) or not self.scale_b11.is_contiguous():
+            self.scale_b11 = torch.empty((B, 1, 1), dtype=torch.float32, device=dev)
+        vals = {}
+        for k, v in comps.items():
+            if isinstance(v, torch.Tensor):
+                t = v if (v.dtype == torch.float32 and v.device == dev) else v.to(dev, torch.float32)
+                t = t.reshape(-1) if t.dim() != 1 else t
+                if t.numel() < B:
+                    if t.numel() != 1:
+                        raise ValueError(f"pose component {k}: {t.numel()} values for {B} rows")
+                    t = t.expand(B)
+                vals[k] = t[:B]
+            else:
+                vals[k] = float(v)
+        if dev.type != "cuda":
+            return {k: (v if isinstance(v, torch.Tensor) else torch.full((B,), v, device=dev))
+                    for k, v in vals.items()}
+        from .. import _abi
+
+        d = _abi.pbr_pose_desc()
+        d.transforms = self.transforms_b44.data_ptr()
+        d.rot3 = self.rot3_b33.data_ptr()
+        d.scale = self.scale_b11.data_ptr()
+        d.rows = B
+        keep = []
+        for k, v in vals.items():
+            c = getattr(d, k)
+            if isinstance(v, torch.Tensor):
+                c.src, c.stride, c.value = v.data_ptr(), v.stride(0), 0.0
+                keep.append(v)
+            else:
+                c.src, c.stride, c.value = None, 0, v
+        return d, keep
+
+    def _pose_host(self, full: dict) -> None:
+        B = self.buf_instances
+        R = type(self)._rotation_mats_from_hpr(torch.stack([full["h"], full["p"], full["r"]], 1))
+        self.rot3_b33[:] = R
+        self.scale_b11[:] = full["s"].reshape(B, 1, 1)
+        self.transforms_b44[:, 0:3, 0:3] = R * self.scale_b11
+        self.transforms_b44[:, 0:3, 3] = torch.stack([full["x"], full["y"], full["z"]], 1)
+        self.transforms_b44[:, 3] = torch.tensor([0.0, 0.0, 0.0, 1.0], device=self.device)
+
     def set_instance_counts(self, counts: torch.Tensor | None) -> None:
         """Active instances per scene ([N] int, clamped to [0, ips]); None = all."""
         if counts is None:

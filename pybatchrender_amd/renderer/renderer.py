@@ -227,6 +227,8 @@ class PBRRenderer:
             raise ValueError(f"out must be a contiguous {tdtype} tensor of shape {(N, C, H, W)}")
         if depth is None:
             depth = bool(self.cfg.output_depth)
+        if vis is None:
+            vis = False
         if depth is True:
             depth = torch.empty((N, H, W), dtype=torch.float32, device=self.device)
         if vis is True:
@@ -247,7 +249,7 @@ class PBRRenderer:
         ws_ptr = 0
         if ws_bytes:
             if self._workspace is None or self._workspace.numel() < ws_bytes:
-                self._workspace = torch.zeros(ws_bytes, dtype=torch.uint8, device=self.device)
+                self._workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
             ws_ptr = self._workspace.data_ptr()
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _native.render(desc, fo, opts, ws_ptr, ws_bytes, stream)
@@ -264,7 +266,28 @@ class PBRRenderer:
         o.region_rows = int(self.cfg.region_rows)
         o.persistent = 1 if self.cfg.persistent else 0
         o.workgroups = 0
+        o.kernel = {"auto": _abi.PBR_KERNEL_AUTO, "chunked": _abi.PBR_KERNEL_CHUNKED,
+                    "scene": _abi.PBR_KERNEL_SCENE}[self.cfg.kernel]
+        o.bin_width = int(self.cfg.bin_width)
+        o.bins_per_item = int(self.cfg.bins_per_item)
+        o.lane_pixels = int(self.cfg.lane_pixels)
         return o
+
+    def set_poses(self, poses) -> None:
+        """PBRNode.set_pose for several nodes, one HIP launch on the GPU.
+        ``poses`` is a sequence of (node, {x, y, z, h, p, r, s}) pairs."""
+        prepared = [(node, node._pose_inputs(**{"x": 0.0, "y": 0.0, "z": 0.0, "h": 0.0, "p": 0.0,
+                                                "r": 0.0, "s": 1.0, **kw}))
+                    for node, kw in poses]
+        prepared = [(n, p) for n, p in prepared if p is not None]
+        if self.device.type == "cuda":
+            for k in range(0, len(prepared), 8):
+                batch = prepared[k:k + 8]
+                _native.set_poses([p[0] for _, p in batch],
+                                  torch.cuda.current_stream(self.device).cuda_stream)
+            return
+        for node, full in prepared:
+            node._pose_host(full)
 
     def _step(self, *args, **kwargs):
         """Subclass hook: map the caller's state into node/camera tensors."""
@@ -300,6 +323,70 @@ class PBRRenderer:
         img = img.reshape(-1, C, self.cfg.tile_resolution[1], self.cfg.tile_resolution[0])
         return img[: self.cfg.num_scenes]
 
+    def capture_step(self, example_state: torch.Tensor | None = None, depth: bool = False,
+                     vis: bool = False) -> "GraphedStep":
+        """``step`` captured once as a hipGraph (state -> pose launches ->
+        render), replayed per call; see GraphedStep."""
+        return GraphedStep(self, example_state, depth=depth, vis=vis)
+
     def destroy(self) -> None:
         self._geometry = None
         self._workspace = None
+
+
+class GraphedStep:
+    """PBRRenderer.step as one captured hipGraph.
+
+    ``g(state)`` copies ``state`` into the graph's static input, replays the
+    graph (the renderer's ``_step`` launches followed by the render launch)
+    and returns {'pixels', 'depth'?, 'vis'?}.  The returned tensors are the
+    graph's static outputs: the next call overwrites them, clone to keep.
+    Scene changes outside ``_step`` (new nodes, a replaced transforms tensor)
+    need a new capture; values written in place are picked up."""
+
+    def __init__(self, renderer: PBRRenderer, example_state: torch.Tensor | None = None,
+                 depth: bool = False, vis: bool = False, warmup: int = 2):
+        if renderer.device.type != "cuda":
+            raise RuntimeError("graph capture needs a GPU device")
+        self.r = renderer
+        dev = renderer.device
+        self.state = None
+        if example_state is not None:
+            self.state = torch.as_tensor(example_state, dtype=torch.float32).to(dev).clone()
+        N, C, H, W = renderer.output_shape()
+        tdtype, _ = _DTYPES[renderer.cfg.output_dtype]
+        self.out = torch.empty((N, C, H, W), dtype=tdtype, device=dev)
+        self.depth = torch.empty((N, H, W), dtype=torch.float32, device=dev) if depth else None
+        self.vis = torch.empty((N, H, W), dtype=torch.int32, device=dev) if vis else None
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                self._body()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body()
+
+    def _body(self) -> None:
+        if self.state is not None:
+            self.r._step(self.state)
+        self.r.render(out=self.out, depth=self.depth if self.depth is not None else False,
+                      vis=self.vis if self.vis is not None else False)
+
+    def outputs(self) -> dict:
+        res = {"pixels": self.out}
+        if self.depth is not None:
+            res["depth"] = self.depth
+        if self.vis is not None:
+            res["vis"] = self.vis
+        return res
+
+    def __call__(self, state: torch.Tensor | None = None) -> dict:
+        if state is not None:
+            if self.state is None:
+                raise ValueError("captured without a state input")
+            if state.data_ptr() != self.state.data_ptr():
+                self.state.copy_(state, non_blocking=True)
+        self.graph.replay()
+        return self.outputs()

@@ -36,14 +36,14 @@ class PBRShaderContext:
         ch, sh = torch.cos(h), torch.sin(h)
         cp, sp = torch.cos(p), torch.sin(p)
         cr, sr = torch.cos(r), torch.sin(r)
-        B = hpr_b3.shape[0]
-        dev = hpr_b3.device
-        zero = torch.zeros((B,), dtype=torch.float32, device=dev)
-        one = torch.ones((B,), dtype=torch.float32, device=dev)
-        Rz = torch.stack([ch, -sh, zero, sh, ch, zero, zero, zero, one], 1).view(B, 3, 3)
-        Ry = torch.stack([cp, zero, sp, zero, one, zero, -sp, zero, cp], 1).view(B, 3, 3)
-        Rx = torch.stack([one, zero, zero, zero, cr, -sr, zero, sr, cr], 1).view(B, 3, 3)
-        return torch.bmm(torch.bmm(Rz, Ry), Rx)
+        # the product Rz @ Ry @ Rx written out (no batched GEMM for 3x3s)
+        chsp, shsp = ch * sp, sh * sp
+        R = torch.stack([
+            ch * cp, chsp * sr - sh * cr, chsp * cr + sh * sr,
+            sh * cp, shsp * sr + ch * cr, shsp * cr - ch * sr,
+            -sp, cp * sr, cp * cr,
+        ], 1)
+        return R.view(-1, 3, 3)
 
     def _register_self(self) -> None:  # pragma: no cover - abstract in the reference
         raise NotImplementedError
