@@ -1002,19 +1002,38 @@ __global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame 
       uint64_t m_any = __ballot(cls != 0);
       const uint64_t m_full = __ballot(cls == 2);
 
-      // (d) depth: records in id order, so a strictly smaller depth wins
+      // (d) depth: records in id order, so a strictly smaller depth wins.
+      // Flat records carry their colour and id into the pixel state here (the
+      // record is uniform across the wave); smooth ones are shaded after the
+      // pass from the winning record index.
       while (m_any) {
         const int k = __ffsll((unsigned long long)m_any) - 1;
         m_any &= m_any - 1;
         const SceneRec& R = L.rec[k];
         const float t = fmaf(R.zy, pyf, R.zc);
+        const uint32_t rid = R.id;
+        uint32_t rcol = 0;
+        float4 rcolf = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!SMOOTH) {
+          if (SHW == 1)
+            rcol = reinterpret_cast<const uint32_t*>(L.shade)[k];
+          else
+            rcolf = *reinterpret_cast<const float4*>(&L.shade[k * SHW]);
+        }
         if ((m_full >> k) & 1ull) {
 #pragma unroll
           for (int i = 0; i < PX; ++i) {
             const float z = clamp_depth(fmaf(R.zx, pxf[i], t));
-            if (z < bd[i]) {
-              bd[i] = z;
-              brec[i] = k;
+            const bool upd = z < bd[i];
+            bd[i] = upd ? z : bd[i];
+            if (SMOOTH) {
+              brec[i] = upd ? k : brec[i];
+            } else {
+              bid[i] = upd ? rid : bid[i];
+              if (SHW == 1)
+                colp[i] = upd ? rcol : colp[i];
+              else
+                colf[i] = upd ? rcolf : colf[i];
             }
           }
         } else {
@@ -1025,9 +1044,16 @@ __global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame 
 #pragma unroll
           for (int i = 0; i < PX; ++i) {
             const float z = clamp_depth(fmaf(R.zx, pxf[i], t));
-            if ((e0 >= 0.0) & (e1 >= 0.0) & (e2 >= 0.0) & (z < bd[i])) {
-              bd[i] = z;
-              brec[i] = k;
+            const bool upd = (e0 >= 0.0) & (e1 >= 0.0) & (e2 >= 0.0) & (z < bd[i]);
+            bd[i] = upd ? z : bd[i];
+            if (SMOOTH) {
+              brec[i] = upd ? k : brec[i];
+            } else {
+              bid[i] = upd ? rid : bid[i];
+              if (SHW == 1)
+                colp[i] = upd ? rcol : colp[i];
+              else
+                colf[i] = upd ? rcolf : colf[i];
             }
             e0 += d0;
             e1 += d1;
@@ -1036,32 +1062,29 @@ __global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame 
         }
       }
 
-      // colour of pixels whose winner changed in this pass
+      // smooth records: shade the pixels whose winner changed in this pass
+      if (SMOOTH) {
 #pragma unroll
-      for (int i = 0; i < PX; ++i) {
-        if (brec[i] >= 0) {
-          const int r = brec[i];
-          bid[i] = L.rec[r].id;
-          const uint32_t fl = L.flags[r];
-          if (SMOOTH && (fl & 8u)) {
-            const SceneRec& R = L.rec[r];
+        for (int i = 0; i < PX; ++i) {
+          if (brec[i] >= 0) {
+            const int r = brec[i];
+            bid[i] = L.rec[r].id;
+            const uint32_t fl = L.flags[r];
+            const float* sh = &L.shade[r * SHW];
             float col[4];
-            shade_smooth(R.A, R.B, R.C, fl, &L.shade[r * SHW], (double)(256 * (x0 + i) + 128),
-                         py, lt, col);
+            if (fl & 8u) {
+              const SceneRec& R = L.rec[r];
+              shade_smooth(R.A, R.B, R.C, fl, sh, (double)(256 * (x0 + i) + 128), py, lt, col);
+            } else {
+              col[0] = sh[0];
+              col[1] = sh[1];
+              col[2] = sh[2];
+              col[3] = sh[3];
+            }
             if (CMODE == 0)
               colp[i] = pack_rgba8(col);
             else
               colf[i] = make_float4(col[0], col[1], col[2], col[3]);
-          } else if (SHW == 1) {
-            colp[i] = reinterpret_cast<const uint32_t*>(L.shade)[r];
-          } else {
-            const float* sh = &L.shade[r * SHW];
-            if (CMODE == 0) {
-              const float c4[4] = {sh[0], sh[1], sh[2], sh[3]};
-              colp[i] = pack_rgba8(c4);
-            } else {
-              colf[i] = make_float4(sh[0], sh[1], sh[2], sh[3]);
-            }
           }
         }
       }
