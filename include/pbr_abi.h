@@ -34,10 +34,11 @@
 extern "C" {
 #endif
 
-#define PBR_ABI_VERSION 1
+#define PBR_ABI_VERSION 2
 #define PBR_MAX_NODES 16
 #define PBR_MAX_MESHES 16
-/* Largest tile edge in pixels; bounds the fixed-point range (see DESIGN). */
+/* Largest tile edge in pixels: window positions in 1/256 pixel then stay
+ * below 2^24 and every edge function value below 2^51 (exact in fp64). */
 #define PBR_MAX_RESOLUTION 4096
 
 enum pbr_status {
@@ -142,6 +143,16 @@ typedef struct pbr_pose_desc {
   pbr_pose_component x, y, z, h, p, r, s;
 } pbr_pose_desc;
 
+/* Gradient outputs of pbr_render_backward.  Every pointer is optional (NULL:
+ * that gradient is not wanted); the kernels ADD into the buffers, so the
+ * caller zeroes them first (torch autograd convention). */
+typedef struct pbr_grads {
+  float* colors[PBR_MAX_NODES]; /* node j: [buf_instances][4] d/d RGBA       */
+  float* vertices;              /* [V][6] d/d (position, normal), model space */
+  float* light;                 /* [10] d/d light_dir[3] (raw, unnormalised),
+                                   light_col[3], ambient[3], strength        */
+} pbr_grads;
+
 /* Render every scene of `scene` into `out` on `stream`.
  * Replaces: renderer.py:377-389 (step: taskMgr.step + grab_pixels +
  * _rearrange_img) together with shaders/basic.vert and shaders/basic.frag.
@@ -164,6 +175,26 @@ int pbr_set_poses(const pbr_pose_desc* poses, int32_t count, void* stream);
  * No counterpart in the reference (GL owned its buffers). */
 int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts,
                         size_t* bytes);
+
+/* Gradients of the colour output with the visibility held fixed: `vis` is
+ * the [N][H][W] triangle-id buffer a pbr_render call wrote for the same
+ * descriptor, `grad_color` the float32 [N][channels][H][W] gradient of the
+ * loss with respect to that call's colour output.  Pixels shade as
+ * basic.frag:31-38 computes them (per-face normal for flat nodes,
+ * perspective-correct interpolated normals for smooth ones, weights from the
+ * unsnapped window positions).  `workspace` holds at least
+ * pbr_backward_workspace_bytes() bytes.
+ * The reference has no backward (Panda3D/GL is not differentiable); this
+ * serves callers that fit scene parameters to images. */
+int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis,
+                        const float* grad_color, int32_t channels,
+                        const pbr_grads* grads, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* Device workspace pbr_render_backward needs (no counterpart in the
+ * reference). */
+int pbr_backward_workspace_bytes(const pbr_scene_desc* scene, int32_t channels,
+                                 const pbr_grads* grads, size_t* bytes);
 
 /* Thread-local message of the last failed call ("" if none).
  * Replaces the reference's swallowed exceptions (renderer.py:340-350). */

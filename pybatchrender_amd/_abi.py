@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-PBR_ABI_VERSION = 1
+PBR_ABI_VERSION = 2
 PBR_MAX_NODES = 16
 PBR_MAX_MESHES = 16
 PBR_MAX_RESOLUTION = 4096
@@ -119,6 +119,18 @@ class pbr_pose_desc(C.Structure):
     ]
 
 
+class pbr_grads(C.Structure):
+    _fields_ = [
+        ("colors", C.c_void_p * PBR_MAX_NODES),
+        ("vertices", C.c_void_p),
+        ("light", C.c_void_p),
+    ]
+
+
+LIGHT_GRAD_FIELDS = ("light_dir", "light_dir", "light_dir", "light_col", "light_col", "light_col",
+                     "ambient", "ambient", "ambient", "strength")
+
+
 def expected_layout() -> list[int]:
     """The facts ``pbr_abi_layout`` reports, computed from the mirror."""
     return [
@@ -141,12 +153,16 @@ def expected_layout() -> list[int]:
         C.sizeof(pbr_pose_desc),
         pbr_pose_desc.x.offset,
         pbr_pose_desc.s.offset,
+        C.sizeof(pbr_grads),
+        pbr_grads.vertices.offset,
     ]
 
 
 # Exported symbols of libpbr_hip.so, as declared in include/pbr_abi.h.
 EXPORTS = (
     "pbr_render",
+    "pbr_render_backward",
+    "pbr_backward_workspace_bytes",
     "pbr_set_pose",
     "pbr_set_poses",
     "pbr_workspace_bytes",
@@ -167,6 +183,24 @@ def declare(lib: C.CDLL) -> C.CDLL:
         C.c_void_p,
     ]
     lib.pbr_render.restype = C.c_int
+    lib.pbr_render_backward.argtypes = [
+        C.POINTER(pbr_scene_desc),
+        C.c_void_p,
+        C.c_void_p,
+        C.c_int32,
+        C.POINTER(pbr_grads),
+        C.c_void_p,
+        C.c_size_t,
+        C.c_void_p,
+    ]
+    lib.pbr_render_backward.restype = C.c_int
+    lib.pbr_backward_workspace_bytes.argtypes = [
+        C.POINTER(pbr_scene_desc),
+        C.c_int32,
+        C.POINTER(pbr_grads),
+        C.POINTER(C.c_size_t),
+    ]
+    lib.pbr_backward_workspace_bytes.restype = C.c_int
     lib.pbr_set_pose.argtypes = [C.POINTER(pbr_pose_desc), C.c_void_p]
     lib.pbr_set_pose.restype = C.c_int
     lib.pbr_set_poses.argtypes = [C.POINTER(pbr_pose_desc), C.c_int32, C.c_void_p]

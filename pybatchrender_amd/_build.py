@@ -7,7 +7,7 @@ import subprocess
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-SRC = PKG / "csrc" / "pbr_render.hip"
+SRCS = [PKG / "csrc" / "pbr_render.hip", PKG / "csrc" / "pbr_backward.hip"]
 OUT = PKG / "libpbr_hip.so"
 
 HIPCC_FLAGS = [
@@ -32,7 +32,7 @@ def hipcc() -> str:
 
 def build_hip(verbose: bool = False) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(PKG.parent / "include"), "-o", str(tmp), str(SRC)]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(PKG.parent / "include"), "-o", str(tmp), *map(str, SRCS)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -62,3 +62,23 @@ def set_poses(descs: list, stream_ptr: int) -> None:
     rc = library().pbr_set_poses(arr, len(descs), ctypes.c_void_p(stream_ptr or None))
     if rc != _abi.PBR_OK:
         raise RuntimeError(f"pbr_set_poses failed (status {rc}): {last_error()}")
+
+
+def backward_workspace_bytes(desc: _abi.pbr_scene_desc, channels: int, grads: _abi.pbr_grads) -> int:
+    n = ctypes.c_size_t(0)
+    rc = library().pbr_backward_workspace_bytes(ctypes.byref(desc), channels, ctypes.byref(grads),
+                                                ctypes.byref(n))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_backward_workspace_bytes failed (status {rc}): {last_error()}")
+    return int(n.value)
+
+
+def render_backward(desc: _abi.pbr_scene_desc, vis_ptr: int, grad_ptr: int, channels: int,
+                    grads: _abi.pbr_grads, workspace_ptr: int, workspace_bytes: int,
+                    stream_ptr: int) -> None:
+    rc = library().pbr_render_backward(
+        ctypes.byref(desc), ctypes.c_void_p(vis_ptr or None), ctypes.c_void_p(grad_ptr or None),
+        channels, ctypes.byref(grads), ctypes.c_void_p(workspace_ptr or None), workspace_bytes,
+        ctypes.c_void_p(stream_ptr or None))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_render_backward failed (status {rc}): {last_error()}")

@@ -1,0 +1,382 @@
+// pbr_device.h - device-side pieces shared by the forward and backward
+// translation units of libpbr_hip.so: vertex fetch and transform (basic.vert),
+// clipping, fixed-point triangle setup, and the Lambert + ambient shading of
+// basic.frag.  Everything lives in an anonymous namespace, so each
+// translation unit gets its own inlined copy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pbr_abi.h"
+
+namespace {
+
+constexpr int kBlock = 256;  // chunked_kernel workgroup
+constexpr int kWaves = kBlock / 64;
+constexpr int kChunk = 64;   // triangle slots per setup round (chunked_kernel)
+constexpr int kMaxPoly = 9;
+constexpr int kSceneSlots = 64;  // scene_kernel: slots per scene and records per pass
+constexpr uint32_t kOne = 0x3f800000u;
+
+struct CVert {
+  float p[4];  // clip position
+  float n[3];  // normalised world normal
+};
+
+struct Light {
+  float dir[3], col[3], amb[3], s;
+};
+
+__device__ __forceinline__ void normalize3(const float v[3], float out[3]) {
+  float d = v[0] * v[0] + v[1] * v[1];
+  d = d + v[2] * v[2];
+  float r = sqrtf(d);
+  if (r > 0.0f) {
+    out[0] = v[0] / r;
+    out[1] = v[1] / r;
+    out[2] = v[2] / r;
+  } else {
+    out[0] = out[1] = out[2] = 0.0f;
+  }
+}
+
+// basic.frag:33-38 with base = 1 (no texture).
+__device__ __forceinline__ void shade(const float n_in[3], const float rgba[4], const Light& L,
+                                      float out[4]) {
+  float nn[3];
+  normalize3(n_in, nn);
+  float ndl = nn[0] * L.dir[0] + nn[1] * L.dir[1];
+  ndl = ndl + nn[2] * L.dir[2];
+  ndl = fmaxf(ndl, 0.0f);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float light = L.amb[c] + ndl * L.col[c];
+    float l = 1.0f * (1.0f - L.s) + light * L.s;
+    out[c] = rgba[c] * l;
+  }
+  out[3] = rgba[3];
+}
+
+__device__ __forceinline__ uint32_t unorm8(float c) {
+  c = fmaxf(c, 0.0f);
+  c = fminf(c, 1.0f);
+  return (uint32_t)floorf(c * 255.0f + 0.5f);
+}
+
+__device__ __forceinline__ uint32_t pack_rgba8(const float c[4]) {
+  return unorm8(c[0]) | (unorm8(c[1]) << 8) | (unorm8(c[2]) << 16) | (unorm8(c[3]) << 24);
+}
+
+__device__ __forceinline__ uint32_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// Depth clamped to [0, 1] with -0 folded to +0 (the contract's clamp).
+__device__ __forceinline__ float clamp_depth(float z) {
+  z = fmaxf(z, 0.0f);
+  z = fminf(z, 1.0f);
+  return z + 0.0f;
+}
+
+__device__ __forceinline__ Light make_light(const pbr_scene_desc& D) {
+  Light lt;
+  normalize3(D.light_dir, lt.dir);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    lt.col[c] = D.light_col[c];
+    lt.amb[c] = D.ambient[c];
+  }
+  lt.s = fminf(fmaxf(D.strength, 0.0f), 1.0f);
+  return lt;
+}
+
+__device__ __forceinline__ float plane_dist(const CVert& v, int plane, float g) {
+  switch (plane) {
+    case 0: return v.p[2] + v.p[3];
+    case 1: return v.p[3] - v.p[2];
+    case 2: return g * v.p[3] - v.p[0];
+    case 3: return g * v.p[3] + v.p[0];
+    case 4: return g * v.p[3] - v.p[1];
+    default: return g * v.p[3] + v.p[1];
+  }
+}
+
+// Sutherland-Hodgman against near, far and the guard band.  Only reached
+// by triangles that straddle a plane; the private arrays live in scratch.
+__device__ __noinline__ int clip_polygon(CVert* poly, float g) {
+  CVert tmp[kMaxPoly];
+  int n = 3;
+  for (int plane = 0; plane < 6 && n > 0; ++plane) {
+    int m = 0;
+    for (int e = 0; e < n; ++e) {
+      const CVert a = poly[e];
+      const CVert b = poly[e + 1 == n ? 0 : e + 1];
+      float da = plane_dist(a, plane, g);
+      float db = plane_dist(b, plane, g);
+      bool ina = da >= 0.0f, inb = db >= 0.0f;
+      if (ina && m < kMaxPoly) tmp[m++] = a;
+      if (ina != inb && m < kMaxPoly) {
+        const CVert vi = ina ? a : b;
+        const CVert vo = ina ? b : a;
+        float di = ina ? da : db;
+        float dout = ina ? db : da;
+        float t = di / (di - dout);
+        CVert r;
+        for (int k = 0; k < 4; ++k) r.p[k] = vi.p[k] + t * (vo.p[k] - vi.p[k]);
+        for (int k = 0; k < 3; ++k) r.n[k] = vi.n[k] + t * (vo.n[k] - vi.n[k]);
+        tmp[m++] = r;
+      }
+    }
+    n = m;
+    for (int k = 0; k < n; ++k) poly[k] = tmp[k];
+  }
+  return n;
+}
+
+// all_in: no plane clips the triangle; all_out: one plane rejects it whole.
+__device__ __forceinline__ void clip_class(const CVert cv[3], float g, bool& all_in,
+                                           bool& all_out) {
+  all_in = true;
+  all_out = false;
+#pragma unroll
+  for (int pl = 0; pl < 6; ++pl) {
+    float d0 = plane_dist(cv[0], pl, g), d1 = plane_dist(cv[1], pl, g),
+          d2 = plane_dist(cv[2], pl, g);
+    all_in = all_in && d0 >= 0.0f && d1 >= 0.0f && d2 >= 0.0f;
+    all_out = all_out || (d0 < 0.0f && d1 < 0.0f && d2 < 0.0f);
+  }
+}
+
+// Exclusive prefix of per-node triangle counts for scene s (node.py:45-46:
+// cnt instances of the node's mesh, counts[] clamped to [0, ips]).
+__device__ void node_table(const pbr_scene_desc& D, int s, uint32_t* base) {
+  uint32_t acc = 0;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    const pbr_node& nd = D.nodes[j];
+    const int ips = nd.instances_per_scene;
+    int cnt = ips;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), ips);
+    base[j] = acc;
+    acc += (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
+  }
+  base[D.num_nodes] = acc;
+}
+
+// (a) slot -> clip-space triangle of scene s, its instance colour and whether
+// the node is faceted.  Slot order = node, instance, triangle (draw order).
+__device__ __forceinline__ void fetch_node_triangle(const pbr_scene_desc& D, int s, int j,
+                                                    uint32_t local, CVert cv[3], float rgba[4],
+                                                    bool& flat) {
+  const pbr_node& nd = D.nodes[j];
+  const pbr_mesh& me = D.meshes[nd.mesh];
+  const uint32_t ntri = (uint32_t)me.num_triangles;
+  const uint32_t i = local / ntri, t = local - i * ntri;
+  const size_t row =
+      nd.shared ? (size_t)i : (size_t)s * (size_t)nd.instances_per_scene + (size_t)i;
+  const float4* M4 = reinterpret_cast<const float4*>(nd.transforms + row * 16);
+  const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2], m3 = M4[3];
+  const float M[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
+                       m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+  const float4 c4 = reinterpret_cast<const float4*>(nd.colors)[row];
+  rgba[0] = c4.x;
+  rgba[1] = c4.y;
+  rgba[2] = c4.z;
+  rgba[3] = c4.w;
+  flat = nd.flat != 0;
+  const float4* V4 = reinterpret_cast<const float4*>(D.view_proj + (size_t)s * 16);
+  const float4 q0 = V4[0], q1 = V4[1], q2 = V4[2], q3 = V4[3];
+  const float VP[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                        q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const int32_t* tri = D.indices + (size_t)(me.index_offset + (int)t) * 3;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float* vv = D.vertices + (size_t)(me.vertex_offset + tri[k]) * 6;
+    float wp[4], nw[3];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      float a = M[rr * 4 + 0] * vv[0] + M[rr * 4 + 1] * vv[1];
+      a = a + M[rr * 4 + 2] * vv[2];
+      wp[rr] = a + M[rr * 4 + 3];
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      float a = VP[rr * 4 + 0] * wp[0] + VP[rr * 4 + 1] * wp[1];
+      a = a + VP[rr * 4 + 2] * wp[2];
+      cv[k].p[rr] = a + VP[rr * 4 + 3] * wp[3];
+    }
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      float a = M[rr * 4 + 0] * vv[3] + M[rr * 4 + 1] * vv[4];
+      nw[rr] = a + M[rr * 4 + 2] * vv[5];
+    }
+    normalize3(nw, cv[k].n);
+  }
+}
+
+__device__ __forceinline__ void fetch_triangle(const pbr_scene_desc& D, int s,
+                                               const uint32_t* base, uint32_t slot,
+                                               CVert cv[3], float rgba[4], bool& flat) {
+  int j = 0;
+  while (slot >= base[j + 1]) ++j;
+  fetch_node_triangle(D, s, j, slot - base[j], cv, rgba, flat);
+}
+
+// Snapped triangle: the fixed-point facts every later stage derives from.
+struct FixTri {
+  int32_t X[3], Y[3];  // window position in 1/256 pixel, GL y up
+  float zx, zy, zc;    // depth plane in pixel units
+  int i0, i1, j0, j1;  // pixel bounding box clipped to the tile rows [jr0, jr1]
+  uint32_t bias;       // bit k: edge k is not top-left (strict inside test)
+  float invw[3];
+};
+
+// (b) snap, cull, bound, fill-rule bias and depth plane of one clip-space
+// triangle; false if it yields no fragment in GL rows [jr0, jr1].
+__device__ __forceinline__ bool setup_fixed(const CVert& v0, const CVert& v1, const CVert& v2,
+                                            int W, int H, int jr0, int jr1, FixTri& T) {
+  const CVert* v[3] = {&v0, &v1, &v2};
+  int64_t X[3], Y[3];
+  float d[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float w = v[k]->p[3];
+    if (!(w > 0.0f)) return false;
+    float nx = v[k]->p[0] / w, ny = v[k]->p[1] / w, nz = v[k]->p[2] / w;
+    float sx = (nx + 1.0f) * (0.5f * (float)W);
+    float sy = (ny + 1.0f) * (0.5f * (float)H);
+    if (!(fabsf(sx) < 65536.0f) || !(fabsf(sy) < 65536.0f)) return false;  // non-finite input
+    X[k] = (int64_t)rintf(sx * 256.0f);
+    Y[k] = (int64_t)rintf(sy * 256.0f);
+    d[k] = nz * 0.5f + 0.5f;
+    T.invw[k] = 1.0f / w;
+  }
+  int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+  if (area <= 0) return false;
+  int64_t xmin = min(X[0], min(X[1], X[2])), xmax = max(X[0], max(X[1], X[2]));
+  int64_t ymin = min(Y[0], min(Y[1], Y[2])), ymax = max(Y[0], max(Y[1], Y[2]));
+  int64_t i0 = (xmin - 128 + 255) >> 8, i1 = (xmax - 128) >> 8;
+  int64_t j0 = (ymin - 128 + 255) >> 8, j1 = (ymax - 128) >> 8;
+  i0 = max(i0, (int64_t)0);
+  i1 = min(i1, (int64_t)(W - 1));
+  j0 = max(j0, (int64_t)jr0);
+  j1 = min(j1, (int64_t)jr1);
+  if (i0 > i1 || j0 > j1) return false;
+  T.i0 = (int)i0;
+  T.i1 = (int)i1;
+  T.j0 = (int)j0;
+  T.j1 = (int)j1;
+  uint32_t bias = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int a = (k + 1) % 3, b = (k + 2) % 3;
+    const int64_t dx = X[b] - X[a], dy = Y[b] - Y[a];
+    const bool tl = (dy < 0) || (dy == 0 && dx < 0);
+    if (!tl) bias |= 1u << k;
+    T.X[k] = (int32_t)X[k];
+    T.Y[k] = (int32_t)Y[k];
+  }
+  T.bias = bias;
+  double x0 = (double)X[0] * (1.0 / 256.0), y0 = (double)Y[0] * (1.0 / 256.0);
+  double ax = (double)X[1] * (1.0 / 256.0) - x0, ay = (double)Y[1] * (1.0 / 256.0) - y0;
+  double bx = (double)X[2] * (1.0 / 256.0) - x0, by = (double)Y[2] * (1.0 / 256.0) - y0;
+  double ar = ax * by - bx * ay;
+  double dz1 = (double)d[1] - (double)d[0], dz2 = (double)d[2] - (double)d[0];
+  double zxd = (dz1 * by - dz2 * ay) / ar;
+  double zyd = (dz2 * ax - dz1 * bx) / ar;
+  double zcd = (double)d[0] - zxd * x0;
+  zcd = zcd - zyd * y0;
+  T.zx = (float)zxd;
+  T.zy = (float)zyd;
+  T.zc = (float)zcd;
+  return true;
+}
+
+// Exact edge functions e_k = A*Px + B*Py + C of a snapped triangle (edge k is
+// opposite vertex k); C carries the fill-rule bias so inside <=> e >= 0.
+__device__ __forceinline__ void edges_from_fixed(const int32_t X[3], const int32_t Y[3],
+                                                 uint32_t bias, double A[3], double B[3],
+                                                 double C[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int a = (k + 1) % 3, b = (k + 2) % 3;
+    const int64_t Ak = (int64_t)Y[a] - (int64_t)Y[b];
+    const int64_t Bk = (int64_t)X[b] - (int64_t)X[a];
+    const int64_t Ck = -(Ak * (int64_t)X[a] + Bk * (int64_t)Y[a]) - (int64_t)((bias >> k) & 1u);
+    A[k] = (double)Ak;
+    B[k] = (double)Bk;
+    C[k] = (double)Ck;
+  }
+}
+
+// Triangle record in registers before it is committed to LDS.
+struct SubTri {
+  double A[3], B[3], C[3];
+  float zx, zy, zc;
+  int i0, i1, j0, j1;
+  uint32_t bias;
+  float sh[16];  // flat: shaded rgba; smooth: n0 n1 n2 (9), invw (3), rgba (4)
+};
+
+// (b) triangle setup of one clip-space sub-triangle; false if it produces no
+// fragment inside GL rows [jr0, jr1] of a W x H tile.
+template <bool SMOOTH>
+__device__ __forceinline__ bool setup_sub(const CVert& v0, const CVert& v1, const CVert& v2,
+                                          int W, int H, int jr0, int jr1, bool flat,
+                                          const float flat_rgba[4], const float rgba[4],
+                                          SubTri& T) {
+  FixTri F;
+  if (!setup_fixed(v0, v1, v2, W, H, jr0, jr1, F)) return false;
+  edges_from_fixed(F.X, F.Y, F.bias, T.A, T.B, T.C);
+  T.zx = F.zx;
+  T.zy = F.zy;
+  T.zc = F.zc;
+  T.i0 = F.i0;
+  T.i1 = F.i1;
+  T.j0 = F.j0;
+  T.j1 = F.j1;
+  T.bias = F.bias;
+  if (!SMOOTH || flat) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) T.sh[c] = flat_rgba[c];
+  } else {
+    const CVert* v[3] = {&v0, &v1, &v2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) T.sh[k * 3 + c] = v[k]->n[c];
+      T.sh[9 + k] = F.invw[k];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) T.sh[12 + c] = rgba[c];
+  }
+  return true;
+}
+
+// Perspective-correct smooth shading of a record at pixel centre (px, py)
+// (subpixel units): weights = unbiased edge value / w of the opposite vertex.
+__device__ __forceinline__ void shade_smooth(const double A[3], const double B[3],
+                                             const double C[3], uint32_t fl, const float* sh,
+                                             double px, double py, const Light& lt,
+                                             float col[4]) {
+  float bw[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double e = fma(A[k], px, fma(B[k], py, C[k]));
+    e = e + (double)((fl >> k) & 1u);
+    bw[k] = (float)e * sh[9 + k];
+  }
+  float n[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float a = bw[0] * sh[c] + bw[1] * sh[3 + c];
+    n[c] = a + bw[2] * sh[6 + c];
+  }
+  const float rgba[4] = {sh[12], sh[13], sh[14], sh[15]};
+  shade(n, rgba, lt, col);
+}
+
+}  // namespace

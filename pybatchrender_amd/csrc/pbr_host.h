@@ -1,0 +1,33 @@
+// pbr_host.h - host helpers shared by the translation units of libpbr_hip.so.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/pbr_abi.h"
+
+namespace pbr_host {
+
+// Thread-local message returned by pbr_last_error().
+extern thread_local char g_err[512];
+
+// Records a formatted message for pbr_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Descriptor checks every entry point shares (sizes, node/mesh references,
+// null and alignment of the pointers a launch will read).
+int validate_scene(const pbr_scene_desc* D);
+
+// validate_scene plus the output descriptor of pbr_render.
+int validate(const pbr_scene_desc* D, const pbr_frame_out* O);
+
+// Triangle slots of the largest scene: sum over nodes of ips * triangles.
+uint64_t max_slots(const pbr_scene_desc* D);
+
+// True if any node is shaded with interpolated normals.
+bool any_smooth(const pbr_scene_desc* D);
+
+// Compute units of the current device (cached per device).
+int device_cus();
+
+}  // namespace pbr_host
