@@ -910,13 +910,13 @@ int validate_scene(const pbr_scene_desc* D) {
 }
 
 int validate(const pbr_scene_desc* D, const pbr_frame_out* O) {
-  if (!O) return fail(PBR_ERR_INVALID, "null output descriptor");
-  const int rc = validate_scene(D);
-  if (rc) return rc;
+  if (!D || !O) return fail(PBR_ERR_INVALID, "null descriptor");
   if (O->channels < 1 || O->channels > 4)
     return fail(PBR_ERR_INVALID, "channels %d outside [1, 4]", O->channels);
   if (O->dtype != PBR_DTYPE_U8 && O->dtype != PBR_DTYPE_BF16 && O->dtype != PBR_DTYPE_F32)
     return fail(PBR_ERR_INVALID, "unknown dtype %d", O->dtype);
+  const int rc = validate_scene(D);
+  if (rc) return rc;
   if (D->num_scenes > 0 && !O->color) return fail(PBR_ERR_INVALID, "null color output");
   return PBR_OK;
 }

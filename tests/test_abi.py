@@ -54,6 +54,28 @@ def test_invalid_descriptor_is_rejected_without_gpu():
     assert "mesh" in _native.last_error()
 
 
+def test_backward_validation_without_gpu():
+    lib = _native.library()
+    d = _abi.pbr_scene_desc()
+    d.num_scenes, d.width, d.height = 2, 32, 32
+    d.view_proj = 64  # never dereferenced: validation fails first
+    g = _abi.pbr_grads()
+    rc = lib.pbr_render_backward(ctypes.byref(d), None, None, 3, None, None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "gradient descriptor" in _native.last_error()
+    rc = lib.pbr_render_backward(ctypes.byref(d), None, None, 5, ctypes.byref(g), None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "channels" in _native.last_error()
+    rc = lib.pbr_render_backward(ctypes.byref(d), None, None, 3, ctypes.byref(g), None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "visibility" in _native.last_error()
+    g.light = 3  # misaligned
+    rc = lib.pbr_render_backward(ctypes.byref(d), 64, 64, 3, ctypes.byref(g), None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "aligned" in _native.last_error()
+    g.light = None
+    rc = lib.pbr_render_backward(ctypes.byref(d), 64, 64, 3, ctypes.byref(g), None, 0, None)
+    assert rc == _abi.PBR_ERR_WORKSPACE
+    # one partial row of 10 light floats per scene, rounded to 256 bytes
+    assert _native.backward_workspace_bytes(d, 3, g) == 256
+
+
 def test_workspace_size():
     d = _abi.pbr_scene_desc()
     o = _abi.pbr_render_opts()
