@@ -100,6 +100,119 @@ def kernel_time_us(r, launches: int = 200) -> float:
     return e0.elapsed_time(e1) * 1000.0 / launches
 
 
+def time_launches_us(launch, launches: int = 200, warm: int = 20) -> float:
+    """Average duration of `launch()` (which enqueues on torch's current
+    stream) from HIP events around back-to-back calls."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    for _ in range(warm):
+        launch()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(launches):
+        launch()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) * 1000.0 / launches
+
+
+def fwd_bwd_leg(n: int, state, launches: int = 100) -> dict:
+    """Differentiable render of the same workload with float32 output:
+    plain forward (colour only) vs forward with the visibility buffer plus
+    pbr_render_backward (gradients for instance colours, light and vertices),
+    all through the C ABI on one stream; and the same pair through autograd
+    (pybatchrender_amd.diff)."""
+    import torch
+
+    from pybatchrender_amd import _abi, _native, diff
+    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda",
+                                              output_dtype="float32"))
+    r.step(state)
+    N, C, H, W = r.output_shape()
+    desc, keep = r.scene_descriptor()
+    opts = r.render_opts()
+    lib = _native.library()
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = torch.empty((N, C, H, W), dtype=torch.float32, device="cuda")
+    vis = torch.empty((N, H, W), dtype=torch.int32, device="cuda")
+    ws_bytes = _native.workspace_bytes(desc, opts)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    plain = _abi.pbr_frame_out()
+    plain.color, plain.channels, plain.dtype = out.data_ptr(), C, _abi.PBR_DTYPE_F32
+    withvis = _abi.pbr_frame_out()
+    withvis.color, withvis.vis, withvis.channels, withvis.dtype = (out.data_ptr(), vis.data_ptr(),
+                                                                   C, _abi.PBR_DTYPE_F32)
+
+    def render(fo):
+        rc = lib.pbr_render(ctypes.byref(desc), ctypes.byref(fo), ctypes.byref(opts),
+                            ctypes.c_void_p(ws.data_ptr()), ws_bytes, sp)
+        if rc:
+            raise RuntimeError(_native.last_error())
+
+    g = _abi.pbr_grads()
+    gcols = [torch.zeros(nd.buf_instances, 4, device="cuda") for nd in r._geometry["nodes"]]
+    gverts = torch.zeros_like(r._geometry["vertices"])
+    glight = torch.zeros(10, device="cuda")
+    for j, t in enumerate(gcols):
+        g.colors[j] = t.data_ptr()
+    g.vertices, g.light = gverts.data_ptr(), glight.data_ptr()
+    gout = torch.randn((N, C, H, W), device="cuda")
+    bws_bytes = _native.backward_workspace_bytes(desc, C, g)
+    bws = torch.empty(max(bws_bytes, 1), dtype=torch.uint8, device="cuda")
+
+    def backward():
+        rc = lib.pbr_render_backward(ctypes.byref(desc), ctypes.c_void_p(vis.data_ptr()),
+                                     ctypes.c_void_p(gout.data_ptr()), C, ctypes.byref(g),
+                                     ctypes.c_void_p(bws.data_ptr()), bws_bytes, sp)
+        if rc:
+            raise RuntimeError(_native.last_error())
+
+    t_plain = time_launches_us(lambda: render(plain), launches)
+    t_fwd = time_launches_us(lambda: render(withvis), launches)
+    t_bwd = time_launches_us(backward, launches)
+    # autograd path: diff.render + backward of <gout, img>
+    p = diff.scene_params(r)
+
+    def step():
+        img = diff.render(r, p)
+        img.backward(gout)
+
+    for _ in range(5):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = 20
+    for _ in range(iters):
+        step()
+    torch.cuda.synchronize()
+    t_auto = (time.perf_counter() - t0) / iters
+    del keep
+    hw = H * W
+    b_plain = N * (hw * C * 4 + 224 + 64)
+    b_fwd = b_plain + N * hw * 4
+    b_bwd = N * (hw * C * 4 + hw * 4 + 224 + 64)
+    return {
+        "workload": f"CartPole cart+pole+ground quad, {N} scenes {W}x{H}, float32 RGB",
+        "plain_fwd_us": round(t_plain, 2),
+        "fwd_with_vis_us": round(t_fwd, 2),
+        "bwd_us": round(t_bwd, 2),
+        "fwd_bwd_over_plain_fwd": round((t_fwd + t_bwd) / t_plain, 3),
+        "fwd_bwd_scene_frames_per_s": round(N / ((t_fwd + t_bwd) * 1e-6), 1),
+        "plain_fwd_scene_frames_per_s": round(N / (t_plain * 1e-6), 1),
+        "hbm_gbs": {"plain_fwd": round(b_plain / t_plain / 1e3, 1),
+                    "fwd_with_vis": round(b_fwd / t_fwd / 1e3, 1),
+                    "bwd": round(b_bwd / t_bwd / 1e3, 1)},
+        "autograd_step_ms": round(t_auto * 1e3, 3),
+        "note": "C-ABI launches back to back on one stream, HIP events; autograd_step = "
+                "diff.render + backward incl. parameter copies and gradient zeroing",
+    }
+
+
 def measured_traffic() -> int | None:
     """HBM bytes per render call of this workload from the committed counter
     profile (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
@@ -183,6 +296,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-env", action="store_true")
+    ap.add_argument("--no-bwd", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the back-to-back render launches (for counter profiles)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -261,6 +375,9 @@ def main() -> int:
     }
     rec["graphed_step"] = {"scene_frames_per_s": round(graphed, 1),
                            "note": "PBRRenderer.capture_step replay incl. state copy"}
+    if not args.no_bwd:
+        log("[bench] fwd+bwd (differentiable render)")
+        rec["fwd_bwd"] = fwd_bwd_leg(n, states[0])
     if not args.no_env:
         log("[bench] env.step CartPole-v0")
         es = env_step_rate(n, 50)

@@ -76,6 +76,7 @@ struct BwdFrame {
   int W, H, C;
   int cap;         // slots per LDS pass
   int compact;     // 1: number the visible slots (more slots than cap)
+  int vec;         // 1: W % 4 == 0 and 16-byte aligned vis / gradient rows
   int nwords;      // bitmap words (compact)
   int vtab;        // vertex table entries in LDS (0: global atomics)
   int stab;        // shared instance rows in LDS (0: global atomics)
@@ -188,12 +189,15 @@ __device__ __forceinline__ void homog(const float p[4], int W, int H, float h[3]
 // no longer cancel for small triangles far from the window origin.
 __device__ __forceinline__ void local_frame(const BTri& T, int W, int H, float h[3][3], float& ox,
                                             float& oy) {
-  int kb = 0;
+  float pb[4] = {T.p[0][0], T.p[0][1], T.p[0][2], T.p[0][3]};
 #pragma unroll
-  for (int k = 1; k < 3; ++k)
-    if (T.p[k][3] > T.p[kb][3]) kb = k;
+  for (int k = 1; k < 3; ++k) {
+    const bool take = T.p[k][3] > pb[3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pb[r] = take ? T.p[k][r] : pb[r];
+  }
   float hb[3];
-  homog(T.p[kb], W, H, hb);
+  homog(pb, W, H, hb);
   const float lim = 4.0f * (float)max(W, H);
   ox = hb[2] > 0.0f ? fminf(fmaxf(hb[0] / hb[2], -lim), lim) : 0.0f;
   oy = hb[2] > 0.0f ? fminf(fmaxf(hb[1] / hb[2], -lim), lim) : 0.0f;
@@ -296,10 +300,15 @@ __device__ __forceinline__ int pass_index(const BwdFrame& F, const BwdLds& S, ui
   return (ci >= 0 && ci < cnt) ? ci : -1;
 }
 
+// SMOOTH: some node shades with interpolated normals (32 accumulators per
+// run, 8-pixel runs); otherwise every slot is flat (4 accumulators, 16-pixel
+// runs).
+template <bool SMOOTH>
 __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, BwdFrame F,
                                                              BwdLight L, BwdGrads G,
                                                              const int32_t* vis,
                                                              const float* gout, float* partial) {
+  constexpr int RUN = SMOOTH ? 8 : kRun;
   extern __shared__ __align__(16) unsigned char smem[];
   const int s = (int)blockIdx.x;
   const int tid = (int)threadIdx.x;
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
   __syncthreads();
   const uint32_t total = S.base[D.num_nodes];
 
-  const int segs = (W + kRun - 1) / kRun;
+  const int segs = (W + RUN - 1) / RUN;
   const int nruns = H * segs;
   const int32_t* vscene = vis + (size_t)s * plane;
   const float* gscene = gout + (size_t)s * (size_t)C * plane;
@@ -359,8 +368,8 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
   int nvis = (int)total;
   if (F.compact) {
     for (int run = tid; run < nruns; run += kBwdBlock) {
-      const int y = run / segs, x0 = (run - y * segs) * kRun;
-      const int n = min(kRun, W - x0);
+      const int y = run / segs, x0 = (run - y * segs) * RUN;
+      const int n = min(RUN, W - x0);
       const int32_t* vr = vscene + (size_t)y * W + x0;
       int32_t prev = -1;
       for (int i = 0; i < n; ++i) {
@@ -423,9 +432,9 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
         while (jj + 1 < D.num_nodes && slot >= S.base[jj + 1]) ++jj;
         return jj;
       }();
-      const bool smooth = D.nodes[j].flat == 0;
+      const bool smooth = SMOOTH && D.nodes[j].flat == 0;
       S.flags[li] = smooth ? 1u : 0u;
-      if (smooth) {
+      if (SMOOTH && smooth) {
         BTri T;
         bwd_fetch(D, s, S.base, slot, T);
         float h[3][3], ox, oy;
@@ -453,18 +462,52 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
 
     // phase 2: pixel runs
     for (int run = tid; run < nruns; run += kBwdBlock) {
-      const int y = run / segs, x0 = (run - y * segs) * kRun;
-      const int n = min(kRun, W - x0);
+      const int y = run / segs, x0 = (run - y * segs) * RUN;
+      const int n = min(RUN, W - x0);
       const int32_t* vr = vscene + (size_t)y * W + x0;
       const float* gr = gscene + (size_t)y * W + x0;
       const float py = (float)(H - 1 - y) + 0.5f;
+      // the run's visibility and output gradient, loaded up front
+      int32_t vv[RUN];
+      float gg[4][RUN];
+      if (F.vec && n == RUN) {
+#pragma unroll
+        for (int q = 0; q < RUN / 4; ++q) {
+          const int4 t = reinterpret_cast<const int4*>(vr)[q];
+          vv[4 * q] = t.x;
+          vv[4 * q + 1] = t.y;
+          vv[4 * q + 2] = t.z;
+          vv[4 * q + 3] = t.w;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+#pragma unroll
+          for (int q = 0; q < RUN / 4; ++q) {
+            float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (c < C) t = reinterpret_cast<const float4*>(gr + (size_t)c * plane)[q];
+            gg[c][4 * q] = t.x;
+            gg[c][4 * q + 1] = t.y;
+            gg[c][4 * q + 2] = t.z;
+            gg[c][4 * q + 3] = t.w;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < RUN; ++i) {
+          vv[i] = i < n ? vr[i] : -1;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            gg[c][i] = (i < n && c < C) ? gr[(size_t)c * plane + i] : 0.0f;
+        }
+      }
       int cur = -1;
       bool cur_smooth = false;
       float a[kAcc];
 #pragma unroll
       for (int q = 0; q < kAcc; ++q) a[q] = 0.0f;
-      for (int i = 0; i < n; ++i) {
-        const int li = pass_index(F, S, total, vr[i], pbase, cnt);
+#pragma unroll
+      for (int i = 0; i < RUN; ++i) {
+        const int li = pass_index(F, S, total, vv[i], pbase, cnt);
         if (li != cur) {
           if (cur >= 0) {
             float* dst = S.acc + (size_t)cur * kAcc;
@@ -479,11 +522,10 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
 #pragma unroll
           for (int q = 0; q < kAcc; ++q) a[q] = 0.0f;
           cur = li;
-          cur_smooth = li >= 0 && (S.flags[li] & 1u);
+          cur_smooth = SMOOTH && li >= 0 && (S.flags[li] & 1u);
         }
         if (cur < 0) continue;
-        float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int c = 0; c < C; ++c) g[c] = gr[(size_t)c * plane + i];
+        const float g[4] = {gg[0][i], gg[1][i], gg[2][i], gg[3][i]};
         if (!cur_smooth) {
 #pragma unroll
           for (int c = 0; c < 4; ++c) a[c] += g[c];
@@ -543,7 +585,7 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
       float gnw[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
       float gc[4];
       float gl[kLightF];
-      if (T.flat) {
+      if (!SMOOTH || T.flat) {
         // every term is linear in the summed output gradient
         const float gs[4] = {a[0], a[1], a[2], a[3]};
         float t[kAcc];
@@ -599,7 +641,7 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
           gv[k][3 + i] = T.M[0 * 4 + i] * gu[0] + T.M[1 * 4 + i] * gu[1] + T.M[2 * 4 + i] * gu[2];
         gv[k][0] = gv[k][1] = gv[k][2] = 0.0f;
       }
-      if (!T.flat) {
+      if (SMOOTH && !T.flat) {
         float h[3][3], ox, oy;
         local_frame(T, W, H, h, ox, oy);
         const float w012 = T.p[0][3] * T.p[1][3] * T.p[2][3];
@@ -799,6 +841,7 @@ int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis, const f
                 P.partial_bytes, workspace_bytes);
   if ((uintptr_t)workspace & 255) return fail(PBR_ERR_WORKSPACE, "workspace not 256-byte aligned");
   g_err[0] = 0;
+  P.F.vec = (scene->width % 4 == 0) && !((uintptr_t)vis & 15) && !((uintptr_t)grad_color & 15);
 
   BwdLight L;
   {
@@ -820,14 +863,20 @@ int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis, const f
   G.light = grads->light;
   hipStream_t st = (hipStream_t)stream;
   float* partial = reinterpret_cast<float*>(workspace);
+  const bool smooth = any_smooth(scene);
+  const void* kfn = smooth ? (const void*)backward_kernel<true> : (const void*)backward_kernel<false>;
   if (P.lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)backward_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
+    hipError_t e =
+        hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
     if (e != hipSuccess)
       return fail(PBR_ERR_LAUNCH, "backward LDS request failed: %s", hipGetErrorString(e));
   }
-  hipLaunchKernelGGL(backward_kernel, dim3(scene->num_scenes), dim3(kBwdBlock), P.lds, st, *scene,
-                     P.F, L, G, vis, grad_color, partial);
+  if (smooth)
+    hipLaunchKernelGGL(backward_kernel<true>, dim3(scene->num_scenes), dim3(kBwdBlock), P.lds, st,
+                       *scene, P.F, L, G, vis, grad_color, partial);
+  else
+    hipLaunchKernelGGL(backward_kernel<false>, dim3(scene->num_scenes), dim3(kBwdBlock), P.lds, st,
+                       *scene, P.F, L, G, vis, grad_color, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "backward launch failed: %s", hipGetErrorString(e));
   const dim3 rgrid((P.F.P + 255) / 256, (scene->num_scenes + kReduceRows - 1) / kReduceRows);

@@ -76,6 +76,15 @@ __device__ __forceinline__ uint32_t bf16_rne(float f) {
 }
 
 // Depth clamped to [0, 1] with -0 folded to +0 (the contract's clamp).
+// dst = take ? src : dst, component by component (a ternary on the whole
+// float4 becomes a select of addresses and sends the array to scratch).
+__device__ __forceinline__ void sel4(bool take, const float4& src, float4& dst) {
+  dst.x = take ? src.x : dst.x;
+  dst.y = take ? src.y : dst.y;
+  dst.z = take ? src.z : dst.z;
+  dst.w = take ? src.w : dst.w;
+}
+
 __device__ __forceinline__ float clamp_depth(float z) {
   z = fmaxf(z, 0.0f);
   z = fminf(z, 1.0f);

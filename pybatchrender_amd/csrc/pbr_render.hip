@@ -668,7 +668,7 @@ __global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame 
               if (SHW == 1)
                 colp[i] = upd ? rcol : colp[i];
               else
-                colf[i] = upd ? rcolf : colf[i];
+                sel4(upd, rcolf, colf[i]);
             }
           }
         } else {
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame 
               if (SHW == 1)
                 colp[i] = upd ? rcol : colp[i];
               else
-                colf[i] = upd ? rcolf : colf[i];
+                sel4(upd, rcolf, colf[i]);
             }
             e0 += d0;
             e1 += d1;
@@ -730,7 +730,9 @@ __global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame 
     if (yo < H && x0 < W) {
       const size_t rowoff = (size_t)yo * (size_t)W + (size_t)x0;
       const bool vec = (W % PX) == 0;
-      for (int c = 0; c < C; ++c) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= C) break;
         const size_t off = ((size_t)s * C + c) * plane + rowoff;
         if (O.dtype == PBR_DTYPE_U8) {
           uint8_t* dst = reinterpret_cast<uint8_t*>(O.color) + off;
