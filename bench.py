@@ -27,6 +27,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -213,6 +214,53 @@ def fwd_bwd_leg(n: int, state, launches: int = 100) -> dict:
     }
 
 
+def env_loop_leg(n: int = 16384, steps: int = 1000) -> dict:
+    """On-device CartPole env loop: per step one hipGraph replay of
+    {pbr_cartpole_step (dynamics, auto-reset, poses), setup, raster}, bf16 RGB
+    + float32 depth, random actions drawn on the device.  Per-step time from
+    HIP events between consecutive replays; throughput from the wall clock."""
+    import torch
+
+    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender_amd.envs.cartpole.loop import CartPoleLoop
+
+    cfg = CartPoleConfig(num_scenes=n, device="cuda", output_dtype="bfloat16", output_depth=True,
+                         theta_threshold_deg=math.degrees(0.21), x_threshold=2.4, max_steps=500,
+                         init_x_range=(-0.05, 0.05), init_x_dot_range=(-0.05, 0.05),
+                         init_theta_range_deg=(-math.degrees(0.05), math.degrees(0.05)),
+                         init_theta_dot_range_deg=(-math.degrees(0.05), math.degrees(0.05)))
+    loop = CartPoleLoop(CartPoleGroundRenderer(cfg), cfg, actions="random", graph=True, seed=0)
+    for _ in range(20):
+        loop.step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    dones = torch.zeros(n, dtype=torch.int64, device="cuda")
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for k in range(steps):
+        loop.graph.replay()
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = sorted(ev[k].elapsed_time(ev[k + 1]) * 1000.0 for k in range(steps))
+    dones += loop.done.to(torch.int64)
+    hw = 64 * 64
+    step_bytes = n * (hw * (3 * 2 + 4) + 2 * 80 + 64 + 16 + 32)
+    return {
+        "workload": f"{n} CartPole scenes 64x64 bf16 RGB + fp32 depth, device dynamics "
+                    "(dt 0.02, force +-10 N, Bernoulli(0.5) actions, reset |theta|>0.21 or "
+                    "|x|>2.4), one hipGraph per step",
+        "steps": steps,
+        "scene_frames_per_s": round(n * steps / wall, 1),
+        "step_us_p50": round(per[len(per) // 2], 2),
+        "step_us_p99": round(per[min(len(per) - 1, int(len(per) * 0.99))], 2),
+        "step_us_mean": round(sum(per) / len(per), 2),
+        "hbm_gbs_at_p50": round(step_bytes / per[len(per) // 2] / 1e3, 1),
+        "episodes_ending_last_step": int(dones.sum().item()),
+    }
+
+
 def measured_traffic() -> int | None:
     """HBM bytes per render call of this workload from the committed counter
     profile (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
@@ -297,6 +345,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-env", action="store_true")
     ap.add_argument("--no-bwd", action="store_true")
+    ap.add_argument("--no-loop", action="store_true")
+    ap.add_argument("--loop-scenes", type=int, default=16384)
+    ap.add_argument("--loop-steps", type=int, default=1000)
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the back-to-back render launches (for counter profiles)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -378,6 +429,9 @@ def main() -> int:
     if not args.no_bwd:
         log("[bench] fwd+bwd (differentiable render)")
         rec["fwd_bwd"] = fwd_bwd_leg(n, states[0])
+    if not args.no_loop:
+        log("[bench] on-device env loop")
+        rec["env_loop"] = env_loop_leg(args.loop_scenes, args.loop_steps)
     if not args.no_env:
         log("[bench] env.step CartPole-v0")
         es = env_step_rate(n, 50)

@@ -153,6 +153,35 @@ typedef struct pbr_grads {
                                    light_col[3], ambient[3], strength        */
 } pbr_grads;
 
+/* CartPole physics and reset ranges (envs/cartpole/config.py:44-62; angles in
+ * radians, as the env converts them at construction, env.py:55-81). */
+typedef struct pbr_cartpole_params {
+  float gravity, masscart, masspole, length;
+  float force_mag, tau;
+  float theta_threshold;      /* |theta| above this ends the episode        */
+  float x_threshold;          /* |x| above this ends the episode            */
+  int32_t max_steps;          /* step_count >= max_steps - 1 ends it too    */
+  int32_t auto_reset;         /* 1: finished scenes restart from a sample   */
+  float init_lo[4];           /* reset sample ranges of x, x_dot, theta,    */
+  float init_hi[4];           /* theta_dot: U(lo, hi)                       */
+  uint64_t seed;              /* random stream key                          */
+} pbr_cartpole_params;
+
+/* Per-scene CartPole buffers, all [num_scenes] (x4 for states). */
+typedef struct pbr_cartpole_batch {
+  float* state;               /* [N][4] observation (x, x_dot, theta,
+                                 theta_dot), updated in place              */
+  float* next_raw;            /* [N][4] written: state after the dynamics,
+                                 before any reset (what the env renders)   */
+  int64_t* step_count;        /* [N] updated in place                       */
+  const int64_t* action;      /* [N] 0/1; NULL: Bernoulli(0.5) draws        */
+  float* reward;              /* optional [N] written                       */
+  uint8_t* done;              /* optional [N] written (bool)                */
+  uint64_t* rng_counter;      /* [N] draws consumed per scene, in/out       */
+  int32_t num_scenes;
+  int32_t reserved;
+} pbr_cartpole_batch;
+
 /* Render every scene of `scene` into `out` on `stream`.
  * Replaces: renderer.py:377-389 (step: taskMgr.step + grab_pixels +
  * _rearrange_img) together with shaders/basic.vert and shaders/basic.frag.
@@ -170,6 +199,25 @@ int pbr_set_pose(const pbr_pose_desc* pose, void* stream);
 
 /* pbr_set_pose for `count` (<= 8) nodes in one launch. */
 int pbr_set_poses(const pbr_pose_desc* poses, int32_t count, void* stream);
+
+/* One CartPole env step of every scene in one launch: the Euler step of the
+ * cart-pole equations, termination, reward and auto-reset
+ * (envs/cartpole/env.py:111-139 _dynamics/_termination/_reward and
+ * :178-218 _step), then the `npose` (<= 8) pose descriptors for the scene's
+ * row, whose component sources may point into `next_raw` (the state the
+ * reference renders, env.py:191-196), so the instance matrices of the frame
+ * are written by the same launch.  A pose descriptor's row i is written by
+ * the thread of scene i, so its sources must read row i of the batch.
+ * Random numbers (actions when `action` is NULL, reset samples) come from a
+ * counter hash of (seed, scene, rng_counter[scene]); the reference draws
+ * them from torch's global generator instead. */
+int pbr_cartpole_step(const pbr_cartpole_params* params, const pbr_cartpole_batch* batch,
+                      const pbr_pose_desc* poses, int32_t npose, void* stream);
+
+/* Samples a fresh state for every scene and zeroes step_count
+ * (envs/cartpole/env.py:95-109 _sample_initial_state, :153-176 _reset). */
+int pbr_cartpole_reset(const pbr_cartpole_params* params, const pbr_cartpole_batch* batch,
+                       void* stream);
 
 /* Device workspace pbr_render needs for this batch and launch shape.
  * No counterpart in the reference (GL owned its buffers). */

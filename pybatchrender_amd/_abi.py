@@ -127,7 +127,39 @@ class pbr_grads(C.Structure):
     ]
 
 
-LIGHT_GRAD_FIELDS = ("light_dir", "light_dir", "light_dir", "light_col", "light_col", "light_col",
+class pbr_cartpole_params(C.Structure):
+    _fields_ = [
+        ("gravity", C.c_float),
+        ("masscart", C.c_float),
+        ("masspole", C.c_float),
+        ("length", C.c_float),
+        ("force_mag", C.c_float),
+        ("tau", C.c_float),
+        ("theta_threshold", C.c_float),
+        ("x_threshold", C.c_float),
+        ("max_steps", C.c_int32),
+        ("auto_reset", C.c_int32),
+        ("init_lo", C.c_float * 4),
+        ("init_hi", C.c_float * 4),
+        ("seed", C.c_uint64),
+    ]
+
+
+class pbr_cartpole_batch(C.Structure):
+    _fields_ = [
+        ("state", C.c_void_p),
+        ("next_raw", C.c_void_p),
+        ("step_count", C.c_void_p),
+        ("action", C.c_void_p),
+        ("reward", C.c_void_p),
+        ("done", C.c_void_p),
+        ("rng_counter", C.c_void_p),
+        ("num_scenes", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+LIGHT_GRAD_FIELDS =("light_dir", "light_dir", "light_dir", "light_col", "light_col", "light_col",
                      "ambient", "ambient", "ambient", "strength")
 
 
@@ -155,6 +187,10 @@ def expected_layout() -> list[int]:
         pbr_pose_desc.s.offset,
         C.sizeof(pbr_grads),
         pbr_grads.vertices.offset,
+        C.sizeof(pbr_cartpole_params),
+        pbr_cartpole_params.seed.offset,
+        C.sizeof(pbr_cartpole_batch),
+        pbr_cartpole_batch.num_scenes.offset,
     ]
 
 
@@ -165,6 +201,8 @@ EXPORTS = (
     "pbr_backward_workspace_bytes",
     "pbr_set_pose",
     "pbr_set_poses",
+    "pbr_cartpole_step",
+    "pbr_cartpole_reset",
     "pbr_workspace_bytes",
     "pbr_last_error",
     "pbr_abi_version",
@@ -205,6 +243,12 @@ def declare(lib: C.CDLL) -> C.CDLL:
     lib.pbr_set_pose.restype = C.c_int
     lib.pbr_set_poses.argtypes = [C.POINTER(pbr_pose_desc), C.c_int32, C.c_void_p]
     lib.pbr_set_poses.restype = C.c_int
+    lib.pbr_cartpole_step.argtypes = [C.POINTER(pbr_cartpole_params), C.POINTER(pbr_cartpole_batch),
+                                      C.POINTER(pbr_pose_desc), C.c_int32, C.c_void_p]
+    lib.pbr_cartpole_step.restype = C.c_int
+    lib.pbr_cartpole_reset.argtypes = [C.POINTER(pbr_cartpole_params),
+                                       C.POINTER(pbr_cartpole_batch), C.c_void_p]
+    lib.pbr_cartpole_reset.restype = C.c_int
     lib.pbr_workspace_bytes.argtypes = [
         C.POINTER(pbr_scene_desc),
         C.POINTER(pbr_render_opts),

@@ -7,7 +7,8 @@ import subprocess
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-SRCS = [PKG / "csrc" / "pbr_render.hip", PKG / "csrc" / "pbr_backward.hip"]
+SRCS = [PKG / "csrc" / "pbr_render.hip", PKG / "csrc" / "pbr_backward.hip",
+        PKG / "csrc" / "pbr_env.hip"]
 OUT = PKG / "libpbr_hip.so"
 
 HIPCC_FLAGS = [

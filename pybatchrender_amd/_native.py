@@ -82,3 +82,20 @@ def render_backward(desc: _abi.pbr_scene_desc, vis_ptr: int, grad_ptr: int, chan
         ctypes.c_void_p(stream_ptr or None))
     if rc != _abi.PBR_OK:
         raise RuntimeError(f"pbr_render_backward failed (status {rc}): {last_error()}")
+
+
+def cartpole_step(params: _abi.pbr_cartpole_params, batch: _abi.pbr_cartpole_batch, poses: list,
+                  stream_ptr: int) -> None:
+    arr = (_abi.pbr_pose_desc * max(len(poses), 1))(*poses)
+    rc = library().pbr_cartpole_step(ctypes.byref(params), ctypes.byref(batch), arr, len(poses),
+                                     ctypes.c_void_p(stream_ptr or None))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_cartpole_step failed (status {rc}): {last_error()}")
+
+
+def cartpole_reset(params: _abi.pbr_cartpole_params, batch: _abi.pbr_cartpole_batch,
+                   stream_ptr: int) -> None:
+    rc = library().pbr_cartpole_reset(ctypes.byref(params), ctypes.byref(batch),
+                                      ctypes.c_void_p(stream_ptr or None))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_cartpole_reset failed (status {rc}): {last_error()}")

@@ -712,17 +712,32 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
 }
 
 // Sums the partial rows of `rows` workgroups: block (x, y) adds entries
-// [256x, 256x + 256) of rows [32y, 32y + 32) into the gradients.
-constexpr int kReduceRows = 32;
+// [64x, 64x + 64) of rows [64y, 64y + 64) into the gradients.  Thread t
+// takes entry 64x + t % 64 of every fourth row (four loads in flight), the
+// four row lanes combine in LDS and one atomic per entry leaves the block.
+constexpr int kReduceRows = 64;
 
 __global__ __launch_bounds__(256) void reduce_kernel(BwdFrame F, BwdGrads G, int rows,
                                                      const float* partial, int nodes,
                                                      pbr_scene_desc D) {
-  const int e = (int)(blockIdx.x * 256 + threadIdx.x);
-  if (e >= F.P) return;
+  __shared__ float part[4][64];
+  const int lane_e = (int)(threadIdx.x & 63), lane_r = (int)(threadIdx.x >> 6);
+  const int e = (int)blockIdx.x * 64 + lane_e;
   const int r0 = (int)blockIdx.y * kReduceRows, r1 = min(rows, r0 + kReduceRows);
-  float sum = 0.0f;
-  for (int r = r0; r < r1; ++r) sum += partial[(size_t)r * F.P + e];
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (e < F.P) {
+    for (int r = r0 + lane_r; r < r1; r += 16) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = r + 4 * q;
+        if (rr < r1) acc[q] += partial[(size_t)rr * F.P + e];
+      }
+    }
+  }
+  part[lane_r][lane_e] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  __syncthreads();
+  if (lane_r != 0 || e >= F.P) return;
+  const float sum = (part[0][lane_e] + part[1][lane_e]) + (part[2][lane_e] + part[3][lane_e]);
   if (sum == 0.0f) return;
   float* dst = nullptr;
   if (e < kLightF) {
@@ -879,7 +894,7 @@ int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis, const f
                        *scene, P.F, L, G, vis, grad_color, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "backward launch failed: %s", hipGetErrorString(e));
-  const dim3 rgrid((P.F.P + 255) / 256, (scene->num_scenes + kReduceRows - 1) / kReduceRows);
+  const dim3 rgrid((P.F.P + 63) / 64, (scene->num_scenes + kReduceRows - 1) / kReduceRows);
   hipLaunchKernelGGL(reduce_kernel, rgrid, dim3(256), 0, st, P.F, G, scene->num_scenes, partial,
                      scene->num_nodes, *scene);
   e = hipGetLastError();

@@ -388,4 +388,36 @@ __device__ __forceinline__ void shade_smooth(const double A[3], const double B[3
   shade(n, rgba, lt, col);
 }
 
+// ---------------------------------------------------------------------------
+// instance poses: row i of a pose descriptor -> its model matrix
+// M = [[R(h, p, r) * s, (x, y, z)], [0, 0, 0, 1]], R = Rz(h) Ry(p) Rx(r)
+// (node.py:116-154 with shader_context.py:48-84).
+// ---------------------------------------------------------------------------
+constexpr int kMaxPoseBatch = 8;
+
+__device__ __forceinline__ float pose_value(const pbr_pose_component& c, int64_t i) {
+  return c.src ? c.src[i * c.stride] : c.value;
+}
+
+__device__ __forceinline__ void write_pose(const pbr_pose_desc& P, int64_t i) {
+  const float x = pose_value(P.x, i), y = pose_value(P.y, i), z = pose_value(P.z, i);
+  const float h = pose_value(P.h, i), p = pose_value(P.p, i), r = pose_value(P.r, i);
+  const float sc = pose_value(P.s, i);
+  const float ch = cosf(h), sh = sinf(h), cp = cosf(p), sp = sinf(p), cr = cosf(r), sr = sinf(r);
+  const float chsp = ch * sp, shsp = sh * sp;
+  const float R[9] = {ch * cp, chsp * sr - sh * cr, chsp * cr + sh * sr,
+                      sh * cp, shsp * sr + ch * cr, shsp * cr - ch * sr,
+                      -sp,     cp * sr,             cp * cr};
+  float4* M = reinterpret_cast<float4*>(P.transforms + i * 16);
+  M[0] = make_float4(R[0] * sc, R[1] * sc, R[2] * sc, x);
+  M[1] = make_float4(R[3] * sc, R[4] * sc, R[5] * sc, y);
+  M[2] = make_float4(R[6] * sc, R[7] * sc, R[8] * sc, z);
+  M[3] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (P.rot3) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P.rot3[i * 9 + k] = R[k];
+  }
+  if (P.scale) P.scale[i] = sc;
+}
+
 }  // namespace

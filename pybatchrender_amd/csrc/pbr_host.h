@@ -21,6 +21,10 @@ int validate_scene(const pbr_scene_desc* D);
 // validate_scene plus the output descriptor of pbr_render.
 int validate(const pbr_scene_desc* D, const pbr_frame_out* O);
 
+// Checks up to 8 pose descriptors (rows >= 0, 16-byte aligned transforms);
+// writes the largest row count to *max_rows.
+int check_poses(const pbr_pose_desc* poses, int32_t count, int32_t* max_rows);
+
 // Triangle slots of the largest scene: sum over nodes of ips * triangles.
 uint64_t max_slots(const pbr_scene_desc* D);
 

@@ -816,11 +816,6 @@ __global__ __launch_bounds__(64) void raster_kernel(pbr_frame_out O, SmallFrame 
 // ---------------------------------------------------------------------------
 // pose_kernel: instance poses -> model matrices (one thread per row).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float pose_value(const pbr_pose_component& c, int64_t i) {
-  return c.src ? c.src[i * c.stride] : c.value;
-}
-
-constexpr int kMaxPoseBatch = 8;
 struct PoseBatch {
   pbr_pose_desc d[kMaxPoseBatch];
 };
@@ -830,24 +825,7 @@ __global__ __launch_bounds__(256) void pose_kernel(PoseBatch B) {
   const pbr_pose_desc& P = B.d[blockIdx.y];
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= P.rows) return;
-  const float x = pose_value(P.x, i), y = pose_value(P.y, i), z = pose_value(P.z, i);
-  const float h = pose_value(P.h, i), p = pose_value(P.p, i), r = pose_value(P.r, i);
-  const float sc = pose_value(P.s, i);
-  const float ch = cosf(h), sh = sinf(h), cp = cosf(p), sp = sinf(p), cr = cosf(r), sr = sinf(r);
-  const float chsp = ch * sp, shsp = sh * sp;
-  const float R[9] = {ch * cp, chsp * sr - sh * cr, chsp * cr + sh * sr,
-                      sh * cp, shsp * sr + ch * cr, shsp * cr - ch * sr,
-                      -sp,     cp * sr,             cp * cr};
-  float4* M = reinterpret_cast<float4*>(P.transforms + i * 16);
-  M[0] = make_float4(R[0] * sc, R[1] * sc, R[2] * sc, x);
-  M[1] = make_float4(R[3] * sc, R[4] * sc, R[5] * sc, y);
-  M[2] = make_float4(R[6] * sc, R[7] * sc, R[8] * sc, z);
-  M[3] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-  if (P.rot3) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) P.rot3[i * 9 + k] = R[k];
-  }
-  if (P.scale) P.scale[i] = sc;
+  write_pose(P, i);
 }
 
 // Resets the persistent queue head inside the stream order (graph-safe: a
@@ -920,6 +898,22 @@ int validate(const pbr_scene_desc* D, const pbr_frame_out* O) {
   const int rc = validate_scene(D);
   if (rc) return rc;
   if (D->num_scenes > 0 && !O->color) return fail(PBR_ERR_INVALID, "null color output");
+  return PBR_OK;
+}
+
+int check_poses(const pbr_pose_desc* poses, int32_t count, int32_t* max_rows) {
+  if (count < 0 || count > kMaxPoseBatch)
+    return fail(PBR_ERR_INVALID, "pose count %d outside [0, %d]", count, kMaxPoseBatch);
+  if (count > 0 && !poses) return fail(PBR_ERR_INVALID, "null pose descriptors");
+  int32_t m = 0;
+  for (int k = 0; k < count; ++k) {
+    const pbr_pose_desc& p = poses[k];
+    if (p.rows < 0) return fail(PBR_ERR_INVALID, "pose %d: rows < 0", k);
+    if (p.rows > 0 && (!p.transforms || ((uintptr_t)p.transforms & 15)))
+      return fail(PBR_ERR_INVALID, "pose %d: transforms must be a 16-byte aligned pointer", k);
+    if (p.rows > m) m = p.rows;
+  }
+  if (max_rows) *max_rows = m;
   return PBR_OK;
 }
 
@@ -1192,6 +1186,10 @@ int pbr_abi_layout(int64_t* out, int32_t n) {
       (int64_t)offsetof(pbr_pose_desc, s),
       (int64_t)sizeof(pbr_grads),
       (int64_t)offsetof(pbr_grads, vertices),
+      (int64_t)sizeof(pbr_cartpole_params),
+      (int64_t)offsetof(pbr_cartpole_params, seed),
+      (int64_t)sizeof(pbr_cartpole_batch),
+      (int64_t)offsetof(pbr_cartpole_batch, num_scenes),
   };
   const int32_t total = (int32_t)(sizeof(facts) / sizeof(facts[0]));
   for (int32_t i = 0; i < n && i < total; ++i) out[i] = facts[i];
@@ -1199,20 +1197,12 @@ int pbr_abi_layout(int64_t* out, int32_t n) {
 }
 
 int pbr_set_poses(const pbr_pose_desc* poses, int32_t count, void* stream) {
-  if (count < 0 || count > kMaxPoseBatch)
-    return fail(PBR_ERR_INVALID, "pose count %d outside [0, %d]", count, kMaxPoseBatch);
-  if (count > 0 && !poses) return fail(PBR_ERR_INVALID, "null pose descriptors");
+  int32_t max_rows = 0;
+  const int rc = check_poses(poses, count, &max_rows);
+  if (rc) return rc;
   PoseBatch B;
   memset(&B, 0, sizeof(B));
-  int32_t max_rows = 0;
-  for (int k = 0; k < count; ++k) {
-    const pbr_pose_desc& p = poses[k];
-    if (p.rows < 0) return fail(PBR_ERR_INVALID, "pose %d: rows < 0", k);
-    if (p.rows > 0 && (!p.transforms || ((uintptr_t)p.transforms & 15)))
-      return fail(PBR_ERR_INVALID, "pose %d: transforms must be a 16-byte aligned pointer", k);
-    B.d[k] = p;
-    if (p.rows > max_rows) max_rows = p.rows;
-  }
+  for (int k = 0; k < count; ++k) B.d[k] = poses[k];
   g_err[0] = 0;
   if (max_rows == 0) return PBR_OK;
   const int grid = (max_rows + 255) / 256;

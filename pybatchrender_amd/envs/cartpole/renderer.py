@@ -105,10 +105,14 @@ class CartPoleRenderer(PBRRenderer):
         if state_batch is None:
             return
         st = _state_rows(state_batch, int(self.cfg.num_scenes), self.device)
+        self.set_poses(self.pose_spec(st))
+
+    def pose_spec(self, st: torch.Tensor) -> list:
+        """(node, pose components) of a [N, 4] state, read in place."""
         x = st[:, 0]
-        self.set_poses([(self.cart, dict(x=x, s=self.cart.scale_b11.reshape(-1))),
-                        (self.pole, dict(x=x, y=self.pole_y, p=st[:, 2],
-                                         s=self.pole.scale_b11.reshape(-1)))])
+        return [(self.cart, dict(x=x, s=self.cart.scale_b11.reshape(-1))),
+                (self.pole, dict(x=x, y=self.pole_y, p=st[:, 2],
+                                 s=self.pole.scale_b11.reshape(-1)))]
 
 
 class CartPoleGroundRenderer(PBRRenderer):
@@ -158,6 +162,10 @@ class CartPoleGroundRenderer(PBRRenderer):
         if state_batch is None:
             return
         st = _state_rows(state_batch, int(self.cfg.num_scenes), self.device)
+        self.set_poses(self.pose_spec(st))
+
+    def pose_spec(self, st: torch.Tensor) -> list:
+        """(node, pose components) of a [N, 4] state, read in place."""
         x = st[:, 0]
-        self.set_poses([(self.cart, dict(x=x, z=0.5 * self.CART_SCALE[2])),
-                        (self.pole, dict(x=x, z=self.CART_SCALE[2], p=st[:, 2]))])
+        return [(self.cart, dict(x=x, z=0.5 * self.CART_SCALE[2])),
+                (self.pole, dict(x=x, z=self.CART_SCALE[2], p=st[:, 2]))]
