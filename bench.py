@@ -73,10 +73,15 @@ def kernel_time_us(r, launches: int = 200) -> float:
     from pybatchrender_amd import _abi, _native
 
     N, C, H, W = r.output_shape()
-    out = torch.empty((N, C, H, W), dtype=torch.uint8, device="cuda")
+    tdt, code = {"uint8": (torch.uint8, _abi.PBR_DTYPE_U8),
+                 "bfloat16": (torch.bfloat16, _abi.PBR_DTYPE_BF16),
+                 "float32": (torch.float32, _abi.PBR_DTYPE_F32)}[r.cfg.output_dtype]
+    out = torch.empty((N, C, H, W), dtype=tdt, device="cuda")
+    depth = torch.empty((N, H, W), dtype=torch.float32, device="cuda") if r.cfg.output_depth else None
     desc, keep = r.scene_descriptor()
     fo = _abi.pbr_frame_out()
-    fo.color, fo.channels, fo.dtype = out.data_ptr(), C, _abi.PBR_DTYPE_U8
+    fo.color, fo.channels, fo.dtype = out.data_ptr(), C, code
+    fo.depth = depth.data_ptr() if depth is not None else None
     opts = r.render_opts()
     lib = _native.library()
     ws_bytes = _native.workspace_bytes(desc, opts)

@@ -108,20 +108,27 @@ typedef struct pbr_frame_out {
 /* Launch shape.  Zero fields take library defaults. */
 enum pbr_kernel {
   PBR_KERNEL_AUTO = 0,     /* scene kernel when every scene has <= 64 triangle
-                              slots, else the chunked kernel               */
+                              slots, else the tile kernels                 */
   PBR_KERNEL_CHUNKED = 1,  /* any triangle count; region state in LDS      */
-  PBR_KERNEL_SCENE = 2     /* one wave per scene, <= 64 triangle slots     */
+  PBR_KERNEL_SCENE = 2,    /* one wave per scene, <= 64 triangle slots     */
+  PBR_KERNEL_TILES = 3     /* any triangle count: setup once per triangle
+                              into the workspace, then one workgroup per
+                              (scene, square tile)                         */
 };
 
 typedef struct pbr_render_opts {
   int32_t region_rows;        /* chunked: output rows per work item (x8)    */
-  int32_t persistent;         /* chunked: persistent (scene, region) queue  */
+  int32_t persistent;         /* chunked, tiles: persistent work queue      */
   int32_t workgroups;         /* persistent launch size, 0 = 8 per CU       */
   int32_t kernel;             /* enum pbr_kernel                            */
   int32_t bin_width;          /* scene: 16, 32 or 64 pixel-wide bins        */
   int32_t bins_per_item;      /* scene: bins one work item walks, 0 = auto  */
   int32_t lane_pixels;        /* scene: 4 or 8 pixels per lane, 0 = auto    */
-  int32_t reserved;
+  int32_t tile_size;          /* tiles: 16, 32 or 64 pixel tiles, 0 = 32    */
+  int64_t slot_budget;        /* tiles: upper bound of the batch's triangle
+                                 slots (sum over scenes of active instances
+                                 x triangles); 0 = num_scenes x the largest
+                                 scene's slots.  Sizes the workspace.       */
 } pbr_render_opts;
 
 /* One pose component: value of row i = src ? src[i * stride] : value. */

@@ -78,6 +78,7 @@ class pbr_frame_out(C.Structure):
 PBR_KERNEL_AUTO = 0
 PBR_KERNEL_CHUNKED = 1
 PBR_KERNEL_SCENE = 2
+PBR_KERNEL_TILES = 3
 
 
 class pbr_render_opts(C.Structure):
@@ -89,7 +90,8 @@ class pbr_render_opts(C.Structure):
         ("bin_width", C.c_int32),
         ("bins_per_item", C.c_int32),
         ("lane_pixels", C.c_int32),
-        ("reserved", C.c_int32),
+        ("tile_size", C.c_int32),
+        ("slot_budget", C.c_int64),
     ]
 
 
@@ -181,6 +183,7 @@ def expected_layout() -> list[int]:
         C.sizeof(pbr_render_opts),
         pbr_render_opts.kernel.offset,
         pbr_render_opts.bins_per_item.offset,
+        pbr_render_opts.slot_budget.offset,
         C.sizeof(pbr_pose_component),
         C.sizeof(pbr_pose_desc),
         pbr_pose_desc.x.offset,

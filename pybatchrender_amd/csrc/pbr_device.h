@@ -103,6 +103,22 @@ __device__ __forceinline__ Light make_light(const pbr_scene_desc& D) {
   return lt;
 }
 
+// The same light computed on the host (a launch argument).
+inline Light host_light(const pbr_scene_desc* D) {
+  Light lt;
+  const float* v = D->light_dir;
+  float d = v[0] * v[0] + v[1] * v[1];
+  d = d + v[2] * v[2];
+  const float r = sqrtf(d);
+  for (int c = 0; c < 3; ++c) {
+    lt.dir[c] = r > 0.0f ? v[c] / r : 0.0f;
+    lt.col[c] = D->light_col[c];
+    lt.amb[c] = D->ambient[c];
+  }
+  lt.s = fminf(fmaxf(D->strength, 0.0f), 1.0f);
+  return lt;
+}
+
 __device__ __forceinline__ float plane_dist(const CVert& v, int plane, float g) {
   switch (plane) {
     case 0: return v.p[2] + v.p[3];
@@ -386,6 +402,80 @@ __device__ __forceinline__ void shade_smooth(const double A[3], const double B[3
   }
   const float rgba[4] = {sh[12], sh[13], sh[14], sh[15]};
   shade(n, rgba, lt, col);
+}
+
+// ---------------------------------------------------------------------------
+// Snapped triangle record a setup kernel writes for a raster kernel (64 B).
+// ---------------------------------------------------------------------------
+struct GRec {
+  int32_t X[3], Y[3];      // window position, 1/256 pixel, GL y up
+  float zx, zy, zc;        // depth plane in pixel units
+  uint32_t id;             // scene-local triangle id
+  int16_t i0, i1, j0, j1;  // pixel bounding box; i0 = INT16_MAX when culled
+  uint32_t flags;          // bits 0-2 fill-rule bias of edges 0-2, bit 3 smooth,
+                           // bit 4 clipped (tile path: sub-triangles re-derived)
+  uint32_t color;          // packed RGBA8 of a flat record
+  uint32_t pad[2];
+};
+static_assert(sizeof(GRec) == 64, "GRec is one 64-byte line");
+
+constexpr uint32_t kFlagSmooth = 8u;
+constexpr uint32_t kFlagClipped = 16u;
+
+// Shade floats kept per record beside the GRec array: flat rgba, or for
+// smooth records n0 n1 n2 (9), 1/w (3), rgba (4).
+constexpr int kShadeFloats = 16;
+
+template <bool SMOOTH>
+__device__ __forceinline__ void write_grec(const FixTri& T, uint32_t id, bool flat,
+                                           const float flat_rgba[4], const float rgba[4],
+                                           const CVert& v0, const CVert& v1, const CVert& v2,
+                                           GRec* g, float* sh) {
+  GRec r;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    r.X[k] = T.X[k];
+    r.Y[k] = T.Y[k];
+  }
+  r.zx = T.zx;
+  r.zy = T.zy;
+  r.zc = T.zc;
+  r.id = id;
+  r.i0 = (int16_t)T.i0;
+  r.i1 = (int16_t)T.i1;
+  r.j0 = (int16_t)T.j0;
+  r.j1 = (int16_t)T.j1;
+  const bool smooth = SMOOTH && !flat;
+  r.flags = T.bias | (smooth ? kFlagSmooth : 0u);
+  r.color = smooth ? 0u : pack_rgba8(flat_rgba);
+  r.pad[0] = r.pad[1] = 0;
+  const uint4* src = reinterpret_cast<const uint4*>(&r);
+  uint4* dst = reinterpret_cast<uint4*>(g);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = src[q];
+  if (!sh) return;
+  float4* s4 = reinterpret_cast<float4*>(sh);
+  if (!smooth) {
+    s4[0] = make_float4(flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]);
+  } else {
+    s4[0] = make_float4(v0.n[0], v0.n[1], v0.n[2], v1.n[0]);
+    s4[1] = make_float4(v1.n[1], v1.n[2], v2.n[0], v2.n[1]);
+    s4[2] = make_float4(v2.n[2], T.invw[0], T.invw[1], T.invw[2]);
+    s4[3] = make_float4(rgba[0], rgba[1], rgba[2], rgba[3]);
+  }
+}
+
+__device__ __forceinline__ void write_dead(uint32_t id, GRec* g) {
+  GRec r = {};
+  r.id = id;
+  r.i0 = INT16_MAX;
+  r.i1 = -1;
+  r.j0 = INT16_MAX;
+  r.j1 = -1;
+  const uint4* src = reinterpret_cast<const uint4*>(&r);
+  uint4* dst = reinterpret_cast<uint4*>(g);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = src[q];
 }
 
 // ---------------------------------------------------------------------------

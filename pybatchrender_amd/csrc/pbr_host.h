@@ -1,6 +1,7 @@
 // pbr_host.h - host helpers shared by the translation units of libpbr_hip.so.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -33,5 +34,10 @@ bool any_smooth(const pbr_scene_desc* D);
 
 // Compute units of the current device (cached per device).
 int device_cus();
+
+// The tile path (pbr_tiles.hip): workspace size and launch.
+int tiles_workspace_bytes(const pbr_scene_desc* D, const pbr_render_opts* opts, size_t* bytes);
+int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
+                 void* workspace, size_t workspace_bytes, hipStream_t st);
 
 }  // namespace pbr_host

@@ -330,18 +330,6 @@ __global__ __launch_bounds__(kBlock) void chunked_kernel(pbr_scene_desc D, pbr_f
 // raster_kernel (c)-(e), for scenes of at most 64 triangle slots.
 // ---------------------------------------------------------------------------
 
-// Snapped triangle record passed from setup_kernel to raster_kernel (64 B).
-struct GRec {
-  int32_t X[3], Y[3];      // window position, 1/256 pixel, GL y up
-  float zx, zy, zc;        // depth plane in pixel units
-  uint32_t id;             // scene-local triangle id
-  int16_t i0, i1, j0, j1;  // pixel bounding box; i0 = INT16_MAX when culled
-  uint32_t flags;          // bits 0-2 fill-rule bias of edges 0-2, bit 3 smooth
-  uint32_t color;          // packed RGBA8 of a flat record
-  uint32_t pad[2];
-};
-static_assert(sizeof(GRec) == 64, "GRec is one 64-byte line");
-
 struct SmallFrame {
   int W, H;
   float guard;
@@ -351,61 +339,6 @@ struct SmallFrame {
   int bins_per_item;    // consecutive bins one raster work item walks
   int items_per_scene;
 };
-
-// Shade floats kept per record beside the GRec array: flat rgba, or for
-// smooth records n0 n1 n2 (9), 1/w (3), rgba (4).
-constexpr int kShadeFloats = 16;
-
-template <bool SMOOTH>
-__device__ __forceinline__ void write_grec(const FixTri& T, uint32_t id, bool flat,
-                                           const float flat_rgba[4], const float rgba[4],
-                                           const CVert& v0, const CVert& v1, const CVert& v2,
-                                           GRec* g, float* sh) {
-  GRec r;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    r.X[k] = T.X[k];
-    r.Y[k] = T.Y[k];
-  }
-  r.zx = T.zx;
-  r.zy = T.zy;
-  r.zc = T.zc;
-  r.id = id;
-  r.i0 = (int16_t)T.i0;
-  r.i1 = (int16_t)T.i1;
-  r.j0 = (int16_t)T.j0;
-  r.j1 = (int16_t)T.j1;
-  const bool smooth = SMOOTH && !flat;
-  r.flags = T.bias | (smooth ? 8u : 0u);
-  r.color = smooth ? 0u : pack_rgba8(flat_rgba);
-  r.pad[0] = r.pad[1] = 0;
-  const uint4* src = reinterpret_cast<const uint4*>(&r);
-  uint4* dst = reinterpret_cast<uint4*>(g);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) dst[q] = src[q];
-  float4* s4 = reinterpret_cast<float4*>(sh);
-  if (!smooth) {
-    s4[0] = make_float4(flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]);
-  } else {
-    s4[0] = make_float4(v0.n[0], v0.n[1], v0.n[2], v1.n[0]);
-    s4[1] = make_float4(v1.n[1], v1.n[2], v2.n[0], v2.n[1]);
-    s4[2] = make_float4(v2.n[2], T.invw[0], T.invw[1], T.invw[2]);
-    s4[3] = make_float4(rgba[0], rgba[1], rgba[2], rgba[3]);
-  }
-}
-
-__device__ __forceinline__ void write_dead(uint32_t id, GRec* g) {
-  GRec r = {};
-  r.id = id;
-  r.i0 = INT16_MAX;
-  r.i1 = -1;
-  r.j0 = INT16_MAX;
-  r.j1 = -1;
-  const uint4* src = reinterpret_cast<const uint4*>(&r);
-  uint4* dst = reinterpret_cast<uint4*>(g);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) dst[q] = src[q];
-}
 
 // (a)+(b): lane = (scene, triangle slot); a scene owns `seg` consecutive
 // lanes, so the record numbering is a segmented prefix sum in slot order.
@@ -1037,11 +970,19 @@ SmallPlan small_plan(const pbr_scene_desc* D) {
   return p;
 }
 
-bool use_small(const pbr_scene_desc* D, const pbr_render_opts* opts) {
+// The kernel family a call runs: AUTO takes the scene kernel for scenes of
+// at most 64 triangle slots (unless a persistent queue is asked for) and the
+// tile kernels otherwise.
+int choose_kernel(const pbr_scene_desc* D, const pbr_render_opts* opts) {
   const int kernel = opts ? opts->kernel : PBR_KERNEL_AUTO;
-  if (kernel == PBR_KERNEL_CHUNKED) return false;
-  if (kernel == PBR_KERNEL_SCENE) return true;
-  return max_slots(D) <= (uint64_t)kSceneSlots && !(opts && opts->persistent);
+  if (kernel != PBR_KERNEL_AUTO) return kernel;
+  if (max_slots(D) <= (uint64_t)kSceneSlots && !(opts && opts->persistent))
+    return PBR_KERNEL_SCENE;
+  return PBR_KERNEL_TILES;
+}
+
+bool use_small(const pbr_scene_desc* D, const pbr_render_opts* opts) {
+  return choose_kernel(D, opts) == PBR_KERNEL_SCENE;
 }
 
 template <int PX, int LPR, bool SMOOTH>
@@ -1180,6 +1121,7 @@ int pbr_abi_layout(int64_t* out, int32_t n) {
       (int64_t)sizeof(pbr_render_opts),
       (int64_t)offsetof(pbr_render_opts, kernel),
       (int64_t)offsetof(pbr_render_opts, bins_per_item),
+      (int64_t)offsetof(pbr_render_opts, slot_budget),
       (int64_t)sizeof(pbr_pose_component),
       (int64_t)sizeof(pbr_pose_desc),
       (int64_t)offsetof(pbr_pose_desc, x),
@@ -1226,11 +1168,16 @@ int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts
     if (scene->nodes[j].mesh < 0 || scene->nodes[j].mesh >= PBR_MAX_MESHES)
       return fail(PBR_ERR_INVALID, "node %d: mesh %d out of range", j, scene->nodes[j].mesh);
   g_err[0] = 0;
-  if (use_small(scene, opts))
-    *bytes = small_plan(scene).bytes;
-  else
-    *bytes = (opts && opts->persistent) ? 256 : 0;
-  return PBR_OK;
+  switch (choose_kernel(scene, opts)) {
+    case PBR_KERNEL_SCENE:
+      *bytes = small_plan(scene).bytes;
+      return PBR_OK;
+    case PBR_KERNEL_TILES:
+      return tiles_workspace_bytes(scene, opts, bytes);
+    default:
+      *bytes = (opts && opts->persistent) ? 256 : 0;
+      return PBR_OK;
+  }
 }
 
 int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out, const pbr_render_opts* opts,
@@ -1241,10 +1188,17 @@ int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out, const pbr_
   if (scene->num_scenes == 0) return PBR_OK;
   hipStream_t st = (hipStream_t)stream;
   const int kernel = opts ? opts->kernel : PBR_KERNEL_AUTO;
-  if (kernel != PBR_KERNEL_AUTO && kernel != PBR_KERNEL_CHUNKED && kernel != PBR_KERNEL_SCENE)
+  if (kernel != PBR_KERNEL_AUTO && kernel != PBR_KERNEL_CHUNKED && kernel != PBR_KERNEL_SCENE &&
+      kernel != PBR_KERNEL_TILES)
     return fail(PBR_ERR_INVALID, "unknown kernel %d", kernel);
-  if (use_small(scene, opts)) return launch_small(scene, out, opts, workspace, workspace_bytes, st);
-  return launch_chunked(scene, out, opts, workspace, workspace_bytes, st);
+  switch (choose_kernel(scene, opts)) {
+    case PBR_KERNEL_SCENE:
+      return launch_small(scene, out, opts, workspace, workspace_bytes, st);
+    case PBR_KERNEL_TILES:
+      return launch_tiles(scene, out, opts, workspace, workspace_bytes, st);
+    default:
+      return launch_chunked(scene, out, opts, workspace, workspace_bytes, st);
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,676 @@
+// pbr_tiles.hip - the tile path: any triangle count per scene, each triangle
+// set up exactly once.
+//
+//   tile_prep_kernel    per-scene triangle-slot totals (instance counts x
+//                       mesh triangles) -> exclusive prefix = each scene's
+//                       first slot in the record pool; resets the work queue
+//   tile_setup_kernel   one thread per (scene, slot): transform (basic.vert),
+//                       clip class, snap, cull, depth plane, flat colour ->
+//                       pixel bounding box + 64-byte record + colour in the
+//                       pool (triangles that straddle a clip plane keep the
+//                       union box of their sub-triangles and are re-derived
+//                       where they are drawn)
+//   tile_raster_kernel  one workgroup per (scene, square tile), optionally
+//                       persistent over a work queue: the scene's boxes are
+//                       tested 256 at a time, survivors are compacted with a
+//                       wave ballot into two LDS queues, small ones are
+//                       rasterised by one lane each and large ones by a whole
+//                       wave, and every covered pixel takes the LDS atomicMin
+//                       of (depth bits << 32 | slot << 3 | sub-triangle); the
+//                       minimum is the same winner the in-order strict-less
+//                       depth test picks.  A resolve pass shades the winners
+//                       and writes the tile into the NCHW outputs.
+//
+// The arithmetic (snap, fill rule, depth plane, shading) is the shared code
+// of pbr_device.h, so the output is bit-identical to the other kernels.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../../include/pbr_abi.h"
+#include "pbr_device.h"
+#include "pbr_host.h"
+
+using namespace pbr_host;
+
+namespace {
+
+constexpr int kTileBlock = 256;
+constexpr int kQCap = 512;       // entries per LDS queue
+constexpr int kSmallArea = 48;   // box-in-tile pixels one lane rasterises alone
+constexpr uint32_t kDepthOne = 0x3f800000u;
+
+struct TileFrame {
+  int W, H, ts, ntx, tiles;  // tile edge, tiles per row, tiles per scene
+  float guard;
+  uint32_t max_slots;        // slots of the largest scene
+  int uniform;               // 1: no node has per-scene counts
+  int shw;                   // floats per record in `shades` (4 or 16)
+  uint32_t budget;           // records the pool holds
+  int float_out;             // 1: bf16 / float32 colour output
+};
+
+struct TilePool {
+  uint32_t* hdr;        // [0] set when the slots exceed the budget, [1] queue head
+  uint32_t* scene_off;  // [N + 1] first pool slot of each scene (counts only)
+  short4* box;          // [budget] pixel box i0, i1, j0, j1 (GL rows); dead: i0 > i1
+  GRec* recs;           // [budget]
+  uint32_t* col8;       // [budget] packed RGBA8 of flat records
+  float* shades;        // [budget][shw]
+};
+
+__device__ __forceinline__ void scene_range(const TileFrame& F, const TilePool& P, int s,
+                                            uint32_t& off, uint32_t& n) {
+  if (F.uniform) {
+    const uint64_t o = (uint64_t)s * F.max_slots;
+    off = (uint32_t)min(o, (uint64_t)F.budget);
+    n = (uint32_t)min((uint64_t)F.max_slots, (uint64_t)F.budget - off);
+  } else {
+    off = min(P.scene_off[s], F.budget);
+    const uint32_t end = min(P.scene_off[s + 1], F.budget);
+    n = end > off ? end - off : 0u;
+  }
+}
+
+__global__ __launch_bounds__(1024) void tile_prep_kernel(pbr_scene_desc D, TileFrame F,
+                                                         TilePool P, int compute_offsets) {
+  __shared__ uint32_t part[1024];
+  __shared__ unsigned long long carry;
+  const int tid = (int)threadIdx.x;
+  if (tid == 0) {
+    carry = 0ull;
+    P.hdr[0] = 0u;
+    P.hdr[1] = 0u;
+  }
+  __syncthreads();
+  if (!compute_offsets) return;
+  for (int base = 0; base < D.num_scenes; base += 1024) {
+    const int s = base + tid;
+    uint32_t tot = 0;
+    if (s < D.num_scenes) {
+      for (int j = 0; j < D.num_nodes; ++j) {
+        const pbr_node& nd = D.nodes[j];
+        int cnt = nd.instances_per_scene;
+        if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
+        tot += (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
+      }
+    }
+    part[tid] = tot;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+      const uint32_t t = tid >= d ? part[tid - d] : 0u;
+      __syncthreads();
+      part[tid] += t;
+      __syncthreads();
+    }
+    const unsigned long long c = carry;
+    if (s < D.num_scenes) {
+      const unsigned long long o = c + part[tid] - tot;
+      P.scene_off[s] = o > 0xffffffffull ? 0xffffffffu : (uint32_t)o;
+    }
+    __syncthreads();
+    if (tid == 1023) carry = c + part[1023];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const unsigned long long c = carry;
+    P.scene_off[D.num_scenes] = c > 0xffffffffull ? 0xffffffffu : (uint32_t)c;
+    if (c > (unsigned long long)F.budget) P.hdr[0] = 1u;
+  }
+}
+
+// Pixel box of the sub-triangles of a clipped slot that produce fragments.
+__device__ __forceinline__ bool clipped_box(const CVert cv[3], const TileFrame& F, short4& bb) {
+  CVert poly[kMaxPoly];
+  poly[0] = cv[0];
+  poly[1] = cv[1];
+  poly[2] = cv[2];
+  const int n = clip_polygon(poly, F.guard);
+  int i0 = INT16_MAX, i1 = -1, j0 = INT16_MAX, j1 = -1;
+  for (int q = 1; q + 1 < n; ++q) {
+    FixTri T;
+    if (!setup_fixed(poly[0], poly[q], poly[q + 1], F.W, F.H, 0, F.H - 1, T)) continue;
+    i0 = min(i0, T.i0);
+    i1 = max(i1, T.i1);
+    j0 = min(j0, T.j0);
+    j1 = max(j1, T.j1);
+  }
+  bb = make_short4((short)i0, (short)i1, (short)j0, (short)j1);
+  return i0 <= i1;
+}
+
+// Block b handles slots [256 (b % chunks), +256) of scene b / chunks.
+template <bool SMOOTH>
+__global__ __launch_bounds__(256) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
+                                                         TilePool P, Light lt, uint32_t chunks) {
+  __shared__ uint32_t base[PBR_MAX_NODES + 1];
+  const int s = (int)(blockIdx.x / chunks);
+  const uint32_t slot = (blockIdx.x % chunks) * 256u + threadIdx.x;
+  if (threadIdx.x == 0) node_table(D, s, base);
+  __syncthreads();
+  uint32_t off, n;
+  scene_range(F, P, s, off, n);
+  if (slot >= n) return;
+  const uint32_t g = off + slot;
+  CVert cv[3];
+  float rgba[4];
+  bool flat;
+  fetch_triangle(D, s, base, slot, cv, rgba, flat);
+  float flat_rgba[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (!SMOOTH || flat) shade(cv[0].n, rgba, lt, flat_rgba);
+  bool all_in, all_out;
+  clip_class(cv, F.guard, all_in, all_out);
+  short4 bb = make_short4(INT16_MAX, -1, INT16_MAX, -1);
+  // float shade data only where the resolve reads it: smooth records, and
+  // flat ones when the output is float (uint8 output reads col8)
+  float* sh = ((SMOOTH && !flat) || F.float_out) ? P.shades + (size_t)g * F.shw : nullptr;
+  if (all_in) {
+    FixTri T;
+    if (setup_fixed(cv[0], cv[1], cv[2], F.W, F.H, 0, F.H - 1, T)) {
+      write_grec<SMOOTH>(T, slot, flat, flat_rgba, rgba, cv[0], cv[1], cv[2], P.recs + g, sh);
+      bb = make_short4((short)T.i0, (short)T.i1, (short)T.j0, (short)T.j1);
+    }
+  } else if (!all_out) {
+    if (clipped_box(cv, F, bb)) {
+      GRec r = {};
+      r.id = slot;
+      r.i0 = bb.x;
+      r.i1 = bb.y;
+      r.j0 = bb.z;
+      r.j1 = bb.w;
+      const bool smooth = SMOOTH && !flat;
+      r.flags = kFlagClipped | (smooth ? kFlagSmooth : 0u);
+      r.color = smooth ? 0u : pack_rgba8(flat_rgba);
+      const uint4* src = reinterpret_cast<const uint4*>(&r);
+      uint4* dst = reinterpret_cast<uint4*>(P.recs + g);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dst[q] = src[q];
+      if (!smooth && sh)
+        *reinterpret_cast<float4*>(sh) =
+            make_float4(flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]);
+    }
+  }
+  P.box[g] = bb;
+  if (!F.float_out) P.col8[g] = pack_rgba8(flat_rgba);
+}
+
+struct TileCtx {
+  int s;
+  uint32_t off;
+  int ps;                 // pixel-state row stride (the tile edge)
+  int x0, yo0, w, h;      // tile origin (column, output row) and extent
+  int ti0, ti1, tj0, tj1; // tile rectangle in GL pixel coordinates
+};
+
+// Depth test of one fragment of the tile: LDS atomicMin on the key
+// (depth bits, rank).  Keys only decrease, so a plain read that is already
+// <= the key proves the atomic would change nothing.
+__device__ __forceinline__ void fragment(unsigned long long* pix, const TileCtx& t, int i, int j,
+                                         float z, uint32_t rank) {
+  const int ly = t.tj1 - j;  // output row within the tile (row 0 = top)
+  const int lx = i - t.ti0;
+  const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | rank;
+  unsigned long long* q = pix + ly * t.ps + lx;
+  if (key < *q) atomicMin(q, key);
+}
+
+// Rasterises the snapped triangle T (rank `rank`) over the part of its box
+// inside the tile, one lane walking every pixel.
+__device__ __forceinline__ void raster_lane(unsigned long long* pix, const TileCtx& t,
+                                            const int32_t X[3], const int32_t Y[3],
+                                            uint32_t bias, float zx, float zy, float zc,
+                                            int bi0, int bi1, int bj0, int bj1, uint32_t rank) {
+  double A[3], B[3], Cc[3];
+  edges_from_fixed(X, Y, bias, A, B, Cc);
+  const int a0 = max(bi0, t.ti0), a1 = min(bi1, t.ti1);
+  const int b0 = max(bj0, t.tj0), b1 = min(bj1, t.tj1);
+  for (int j = b0; j <= b1; ++j) {
+    const double py = (double)(256 * j + 128);
+    const double px0 = (double)(256 * a0 + 128);
+    double e0 = fma(A[0], px0, fma(B[0], py, Cc[0]));
+    double e1 = fma(A[1], px0, fma(B[1], py, Cc[1]));
+    double e2 = fma(A[2], px0, fma(B[2], py, Cc[2]));
+    const float pyf = (float)j + 0.5f;
+    const float tz = fmaf(zy, pyf, zc);
+    for (int i = a0; i <= a1; ++i) {
+      if (e0 >= 0.0 && e1 >= 0.0 && e2 >= 0.0) {
+        const float z = clamp_depth(fmaf(zx, (float)i + 0.5f, tz));
+        fragment(pix, t, i, j, z, rank);
+      }
+      e0 += A[0] * 256.0;
+      e1 += A[1] * 256.0;
+      e2 += A[2] * 256.0;
+    }
+  }
+}
+
+// The same over a whole wave: lane k takes pixels k, k + 64, ... of the box.
+__device__ __forceinline__ void raster_wave(unsigned long long* pix, const TileCtx& t, int lane,
+                                            const int32_t X[3], const int32_t Y[3],
+                                            uint32_t bias, float zx, float zy, float zc,
+                                            int bi0, int bi1, int bj0, int bj1, uint32_t rank) {
+  double A[3], B[3], Cc[3];
+  edges_from_fixed(X, Y, bias, A, B, Cc);
+  const int a0 = max(bi0, t.ti0), a1 = min(bi1, t.ti1);
+  const int b0 = max(bj0, t.tj0), b1 = min(bj1, t.tj1);
+  if (a0 > a1 || b0 > b1) return;
+  const int w = a1 - a0 + 1;
+  const int npx = w * (b1 - b0 + 1);
+  for (int p = lane; p < npx; p += 64) {
+    const int i = a0 + p % w, j = b0 + p / w;
+    const double px = (double)(256 * i + 128), py = (double)(256 * j + 128);
+    const double e0 = fma(A[0], px, fma(B[0], py, Cc[0]));
+    const double e1 = fma(A[1], px, fma(B[1], py, Cc[1]));
+    const double e2 = fma(A[2], px, fma(B[2], py, Cc[2]));
+    if (e0 >= 0.0 && e1 >= 0.0 && e2 >= 0.0) {
+      const float z = clamp_depth(fmaf(zx, (float)i + 0.5f, fmaf(zy, (float)j + 0.5f, zc)));
+      fragment(pix, t, i, j, z, rank);
+    }
+  }
+}
+
+// Sub-triangle q of a clipped slot, re-derived exactly as the setup did.
+__device__ __forceinline__ bool clipped_sub(const pbr_scene_desc& D, const TileFrame& F, int s,
+                                            const uint32_t* base, uint32_t slot, int q,
+                                            FixTri& T, CVert sub[3]) {
+  CVert cv[3];
+  float rgba[4];
+  bool flat;
+  fetch_triangle(D, s, base, slot, cv, rgba, flat);
+  CVert poly[kMaxPoly];
+  poly[0] = cv[0];
+  poly[1] = cv[1];
+  poly[2] = cv[2];
+  const int n = clip_polygon(poly, F.guard);
+  if (q < 1 || q + 1 >= n) return false;
+  sub[0] = poly[0];
+  sub[1] = poly[q];
+  sub[2] = poly[q + 1];
+  return setup_fixed(poly[0], poly[q], poly[q + 1], F.W, F.H, 0, F.H - 1, T);
+}
+
+// Every fragment-producing sub-triangle of a clipped slot; `lane` < 0 means
+// one lane does all the pixels, otherwise the wave shares them.
+__device__ __noinline__ void raster_clipped(unsigned long long* pix, const TileCtx& t,
+                                            const pbr_scene_desc& D, const TileFrame& F,
+                                            const uint32_t* base, uint32_t slot, int lane) {
+  CVert cv[3];
+  float rgba[4];
+  bool flat;
+  fetch_triangle(D, t.s, base, slot, cv, rgba, flat);
+  CVert poly[kMaxPoly];
+  poly[0] = cv[0];
+  poly[1] = cv[1];
+  poly[2] = cv[2];
+  const int n = clip_polygon(poly, F.guard);
+  for (int q = 1; q + 1 < n; ++q) {
+    FixTri T;
+    if (!setup_fixed(poly[0], poly[q], poly[q + 1], F.W, F.H, 0, F.H - 1, T)) continue;
+    const uint32_t rank = (slot << 3) | (uint32_t)(q - 1);
+    if (lane < 0)
+      raster_lane(pix, t, T.X, T.Y, T.bias, T.zx, T.zy, T.zc, T.i0, T.i1, T.j0, T.j1, rank);
+    else
+      raster_wave(pix, t, lane, T.X, T.Y, T.bias, T.zx, T.zy, T.zc, T.i0, T.i1, T.j0, T.j1,
+                  rank);
+  }
+}
+
+__device__ __forceinline__ void load_grec(const GRec* g, GRec& r) {
+  const uint4* src = reinterpret_cast<const uint4*>(g);
+  uint4* dst = reinterpret_cast<uint4*>(&r);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = src[q];
+}
+
+template <bool SMOOTH, int CMODE, int TS>
+__global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc D,
+                                                                 pbr_frame_out O, TileFrame F,
+                                                                 TilePool P, Light lt, float4 bg,
+                                                                 uint32_t total_items,
+                                                                 int persistent) {
+  __shared__ unsigned long long pix[TS * TS];
+  __shared__ uint32_t qs[kQCap], qb[kQCap];
+  __shared__ uint32_t ns, nb, item_sh;
+  __shared__ uint32_t base[PBR_MAX_NODES + 1];
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int W = F.W, H = F.H;
+  const float bgc[4] = {bg.x, bg.y, bg.z, bg.w};
+
+  uint32_t item = blockIdx.x;
+  if (persistent) {
+    if (tid == 0) item_sh = atomicAdd(&P.hdr[1], 1u);
+    __syncthreads();
+    item = item_sh;
+  }
+  while (item < total_items) {
+    TileCtx t;
+    t.ps = TS;
+    t.s = (int)(item / (uint32_t)F.tiles);
+    const int tt = (int)(item % (uint32_t)F.tiles);
+    t.x0 = (tt % F.ntx) * TS;
+    t.yo0 = (tt / F.ntx) * TS;
+    t.w = min(TS, W - t.x0);
+    t.h = min(TS, H - t.yo0);
+    t.ti0 = t.x0;
+    t.ti1 = t.x0 + t.w - 1;
+    t.tj1 = H - 1 - t.yo0;
+    t.tj0 = t.tj1 - (t.h - 1);
+    uint32_t n;
+    scene_range(F, P, t.s, t.off, n);
+    for (int p = tid; p < TS * TS; p += kTileBlock)
+      pix[p] = (unsigned long long)kDepthOne << 32;
+    if (tid == 0) {
+      ns = 0u;
+      nb = 0u;
+      node_table(D, t.s, base);
+    }
+    __syncthreads();
+
+    for (uint32_t r0 = 0; r0 < n; r0 += kTileBlock) {
+      const uint32_t slot = r0 + (uint32_t)tid;
+      bool small = false, big = false;
+      if (slot < n) {
+        const short4 b = P.box[t.off + slot];
+        const int a0 = max((int)b.x, t.ti0), a1 = min((int)b.y, t.ti1);
+        const int c0 = max((int)b.z, t.tj0), c1 = min((int)b.w, t.tj1);
+        if (a0 <= a1 && c0 <= c1) {
+          small = (a1 - a0 + 1) * (c1 - c0 + 1) <= kSmallArea;
+          big = !small;
+        }
+      }
+      // wave-aggregated enqueue (ballot + prefix within the wave)
+      const uint64_t below = (1ull << lane) - 1ull;
+      const uint64_t ms = __ballot(small), mb = __ballot(big);
+      uint32_t bs = 0, bbq = 0;
+      if (lane == 0) {
+        if (ms) bs = atomicAdd(&ns, (uint32_t)__popcll(ms));
+        if (mb) bbq = atomicAdd(&nb, (uint32_t)__popcll(mb));
+      }
+      bs = __shfl(bs, 0);
+      bbq = __shfl(bbq, 0);
+      if (small) qs[bs + (uint32_t)__popcll(ms & below)] = slot;
+      if (big) qb[bbq + (uint32_t)__popcll(mb & below)] = slot;
+      __syncthreads();
+      const bool last = r0 + kTileBlock >= n;
+      const uint32_t cs = ns, cb = nb;
+      const bool flush_s = cs > (uint32_t)(kQCap - kTileBlock) || (last && cs > 0);
+      const bool flush_b = cb > (uint32_t)(kQCap - kTileBlock) || (last && cb > 0);
+      if (flush_s) {
+        for (uint32_t k = (uint32_t)tid; k < cs; k += kTileBlock) {
+          const uint32_t sl = qs[k];
+          GRec R;
+          load_grec(P.recs + t.off + sl, R);
+          if (R.flags & kFlagClipped)
+            raster_clipped(pix, t, D, F, base, sl, -1);
+          else
+            raster_lane(pix, t, R.X, R.Y, R.flags & 7u, R.zx, R.zy, R.zc, R.i0, R.i1, R.j0,
+                        R.j1, sl << 3);
+        }
+      }
+      if (flush_b) {
+        for (uint32_t k = (uint32_t)wave; k < cb; k += kTileBlock / 64) {
+          const uint32_t sl = qb[k];
+          GRec R;
+          load_grec(P.recs + t.off + sl, R);
+          if (R.flags & kFlagClipped)
+            raster_clipped(pix, t, D, F, base, sl, lane);
+          else
+            raster_wave(pix, t, lane, R.X, R.Y, R.flags & 7u, R.zx, R.zy, R.zc, R.i0, R.i1,
+                        R.j0, R.j1, sl << 3);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        if (flush_s) ns = 0u;
+        if (flush_b) nb = 0u;
+      }
+      __syncthreads();
+    }
+
+    // resolve: shade each pixel's winner, write the tile
+    const int C = O.channels;
+    const size_t plane = (size_t)H * (size_t)W;
+    for (int p = tid; p < t.w * t.h; p += kTileBlock) {
+      const int ly = p / t.w, lx = p - ly * t.w;
+      const unsigned long long v = pix[ly * TS + lx];
+      const uint32_t zb = (uint32_t)(v >> 32);
+      const uint32_t rank = (uint32_t)v;
+      const bool covered = zb < kDepthOne;
+      const int x = t.x0 + lx, yo = t.yo0 + ly;
+      const uint32_t sl = rank >> 3;
+      const size_t g = (size_t)t.off + sl;
+      float col[4] = {bgc[0], bgc[1], bgc[2], bgc[3]};
+      uint32_t packed = 0;
+      bool have_packed = false;
+      if (covered) {
+        bool smooth = false;
+        if (SMOOTH) smooth = (P.recs[g].flags & kFlagSmooth) != 0u;
+        if (!smooth) {
+          if (CMODE == 0) {
+            packed = P.col8[g];
+            have_packed = true;
+          } else {
+            const float4 c4 = *reinterpret_cast<const float4*>(P.shades + g * F.shw);
+            col[0] = c4.x;
+            col[1] = c4.y;
+            col[2] = c4.z;
+            col[3] = c4.w;
+          }
+        } else {
+          GRec R;
+          load_grec(P.recs + g, R);
+          const int jg = H - 1 - yo;
+          const double px = (double)(256 * x + 128), py = (double)(256 * jg + 128);
+          if (R.flags & kFlagClipped) {
+            FixTri T;
+            CVert sub[3];
+            if (clipped_sub(D, F, t.s, base, sl, (int)(rank & 7u) + 1, T, sub)) {
+              double A[3], B[3], Cc[3];
+              edges_from_fixed(T.X, T.Y, T.bias, A, B, Cc);
+              float sh[16];
+              CVert cv[3];
+              float rgba[4];
+              bool flat;
+              fetch_triangle(D, t.s, base, sl, cv, rgba, flat);
+#pragma unroll
+              for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sh[3 * k + c] = sub[k].n[c];
+                sh[9 + k] = T.invw[k];
+              }
+#pragma unroll
+              for (int c = 0; c < 4; ++c) sh[12 + c] = rgba[c];
+              shade_smooth(A, B, Cc, T.bias, sh, px, py, lt, col);
+            }
+          } else {
+            double A[3], B[3], Cc[3];
+            edges_from_fixed(R.X, R.Y, R.flags & 7u, A, B, Cc);
+            shade_smooth(A, B, Cc, R.flags & 7u, P.shades + g * F.shw, px, py, lt, col);
+          }
+        }
+      }
+      if (CMODE == 0 && !have_packed) packed = pack_rgba8(col);
+      const size_t pidx = (size_t)yo * W + x;
+      for (int c = 0; c < C; ++c) {
+        const size_t o = ((size_t)t.s * C + c) * plane + pidx;
+        if (O.dtype == PBR_DTYPE_U8) {
+          reinterpret_cast<uint8_t*>(O.color)[o] = (uint8_t)((packed >> (8 * c)) & 255u);
+        } else {
+          const float vv = col[c];
+          if (O.dtype == PBR_DTYPE_F32)
+            reinterpret_cast<float*>(O.color)[o] = vv;
+          else
+            reinterpret_cast<uint16_t*>(O.color)[o] = (uint16_t)bf16_rne(vv);
+        }
+      }
+      const size_t dofs = (size_t)t.s * plane + pidx;
+      if (O.depth) O.depth[dofs] = covered ? __uint_as_float(zb) : 1.0f;
+      if (O.vis) O.vis[dofs] = covered ? (int32_t)sl : -1;
+    }
+
+    if (!persistent) break;
+    __syncthreads();
+    if (tid == 0) item_sh = atomicAdd(&P.hdr[1], 1u);
+    __syncthreads();
+    item = item_sh;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kMaxTileSlots = 1ull << 29;  // the rank packs slot << 3
+
+struct TilePlan {
+  TileFrame F;
+  bool counts;
+  bool smooth;
+  size_t off_scene, off_box, off_recs, off_col8, off_shades, bytes;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+int tiles_plan(const pbr_scene_desc* D, const pbr_render_opts* opts, TilePlan& T) {
+  memset(&T, 0, sizeof(T));
+  const int ts = (opts && opts->tile_size > 0) ? opts->tile_size : 32;
+  if (ts != 16 && ts != 32 && ts != 64)
+    return fail(PBR_ERR_INVALID, "tile_size %d is not 16, 32 or 64", ts);
+  const uint64_t ms = max_slots(D);
+  if (ms >= kMaxTileSlots)
+    return fail(PBR_ERR_UNSUPPORTED, "tile path: %llu triangle slots per scene (limit 2^29)",
+                (unsigned long long)ms);
+  uint64_t budget = (opts && opts->slot_budget > 0) ? (uint64_t)opts->slot_budget
+                                                     : (uint64_t)D->num_scenes * ms;
+  if (budget >= 0xffffffffull)
+    return fail(PBR_ERR_UNSUPPORTED, "tile path: slot budget %llu exceeds 2^32 - 1",
+                (unsigned long long)budget);
+  T.counts = false;
+  for (int j = 0; j < D->num_nodes; ++j)
+    if (D->nodes[j].counts) T.counts = true;
+  T.smooth = any_smooth(D);
+  TileFrame& F = T.F;
+  F.W = D->width;
+  F.H = D->height;
+  F.ts = ts;
+  F.ntx = (D->width + ts - 1) / ts;
+  F.tiles = F.ntx * ((D->height + ts - 1) / ts);
+  F.guard = 65536.0f / (float)(D->width > D->height ? D->width : D->height);
+  F.max_slots = (uint32_t)ms;
+  F.uniform = T.counts ? 0 : 1;
+  F.shw = T.smooth ? kShadeFloats : 4;
+  F.budget = (uint32_t)budget;
+  size_t o = 256;  // header
+  T.off_scene = o;
+  o = align256(o + 4 * ((size_t)D->num_scenes + 1));
+  T.off_box = o;
+  o = align256(o + 8 * (size_t)budget);
+  T.off_recs = o;
+  o = align256(o + sizeof(GRec) * (size_t)budget);
+  T.off_col8 = o;
+  o = align256(o + 4 * (size_t)budget);
+  T.off_shades = o;
+  o = align256(o + 4 * (size_t)F.shw * (size_t)budget);
+  T.bytes = o;
+  return PBR_OK;
+}
+
+template <bool SMOOTH, int CMODE>
+hipError_t launch_tile_raster(const pbr_scene_desc* D, const pbr_frame_out* O, const TileFrame& F,
+                              const TilePool& P, const Light& lt, float4 bg, uint32_t items,
+                              int grid, int persistent, hipStream_t st) {
+  switch (F.ts) {
+    case 16:
+      hipLaunchKernelGGL((tile_raster_kernel<SMOOTH, CMODE, 16>), dim3(grid), dim3(kTileBlock), 0,
+                         st, *D, *O, F, P, lt, bg, items, persistent);
+      break;
+    case 64:
+      hipLaunchKernelGGL((tile_raster_kernel<SMOOTH, CMODE, 64>), dim3(grid), dim3(kTileBlock), 0,
+                         st, *D, *O, F, P, lt, bg, items, persistent);
+      break;
+    default:
+      hipLaunchKernelGGL((tile_raster_kernel<SMOOTH, CMODE, 32>), dim3(grid), dim3(kTileBlock), 0,
+                         st, *D, *O, F, P, lt, bg, items, persistent);
+      break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace pbr_host {
+
+int tiles_workspace_bytes(const pbr_scene_desc* D, const pbr_render_opts* opts, size_t* bytes) {
+  TilePlan T;
+  const int rc = tiles_plan(D, opts, T);
+  if (rc) return rc;
+  *bytes = D->num_scenes > 0 ? T.bytes : 0;
+  return PBR_OK;
+}
+
+int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
+                 void* workspace, size_t workspace_bytes, hipStream_t st) {
+  TilePlan T;
+  int rc = tiles_plan(D, opts, T);
+  if (rc) return rc;
+  if (!workspace || workspace_bytes < T.bytes)
+    return fail(PBR_ERR_WORKSPACE, "tile path needs a %zu-byte workspace (got %zu)", T.bytes,
+                workspace_bytes);
+  if ((uintptr_t)workspace & 255) return fail(PBR_ERR_WORKSPACE, "workspace not 256-byte aligned");
+  unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
+  TilePool P;
+  P.hdr = reinterpret_cast<uint32_t*>(ws);
+  P.scene_off = reinterpret_cast<uint32_t*>(ws + T.off_scene);
+  P.box = reinterpret_cast<short4*>(ws + T.off_box);
+  P.recs = reinterpret_cast<GRec*>(ws + T.off_recs);
+  P.col8 = reinterpret_cast<uint32_t*>(ws + T.off_col8);
+  P.shades = reinterpret_cast<float*>(ws + T.off_shades);
+  T.F.float_out = O->dtype == PBR_DTYPE_U8 ? 0 : 1;
+  const TileFrame& F = T.F;
+  const int persistent = (opts && opts->persistent) ? 1 : 0;
+  const Light lt = host_light(D);
+  hipError_t e = hipSuccess;
+  if (T.counts || persistent) {
+    hipLaunchKernelGGL(tile_prep_kernel, dim3(1), dim3(1024), 0, st, *D, F, P, T.counts ? 1 : 0);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile prep launch failed: %s", hipGetErrorString(e));
+  }
+  const uint32_t chunks = (F.max_slots + 255) / 256;
+  const uint64_t sblocks = (uint64_t)chunks * (uint64_t)D->num_scenes;
+  if (sblocks >= 0xffffffffull) return fail(PBR_ERR_UNSUPPORTED, "too many setup blocks");
+  if (sblocks > 0) {
+    if (T.smooth)
+      hipLaunchKernelGGL(tile_setup_kernel<true>, dim3((uint32_t)sblocks), dim3(256), 0, st, *D,
+                         F, P, lt, chunks);
+    else
+      hipLaunchKernelGGL(tile_setup_kernel<false>, dim3((uint32_t)sblocks), dim3(256), 0, st, *D,
+                         F, P, lt, chunks);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile setup launch failed: %s", hipGetErrorString(e));
+  }
+  const uint64_t items64 = (uint64_t)D->num_scenes * (uint64_t)F.tiles;
+  if (items64 >= 0x7fffffffull) return fail(PBR_ERR_INVALID, "too many tiles");
+  const uint32_t items = (uint32_t)items64;
+  int grid = (int)items;
+  if (persistent) {
+    const int want = opts->workgroups > 0 ? opts->workgroups : 8 * device_cus();
+    grid = (int)((uint32_t)want < items ? (uint32_t)want : items);
+  }
+  const float4 bg = make_float4(D->background[0], D->background[1], D->background[2],
+                                D->background[3]);
+  const int cmode = O->dtype == PBR_DTYPE_U8 ? 0 : 1;
+  if (T.smooth)
+    e = cmode == 0 ? launch_tile_raster<true, 0>(D, O, F, P, lt, bg, items, grid, persistent, st)
+                   : launch_tile_raster<true, 1>(D, O, F, P, lt, bg, items, grid, persistent, st);
+  else
+    e = cmode == 0 ? launch_tile_raster<false, 0>(D, O, F, P, lt, bg, items, grid, persistent, st)
+                   : launch_tile_raster<false, 1>(D, O, F, P, lt, bg, items, grid, persistent, st);
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile raster launch failed: %s", hipGetErrorString(e));
+  return PBR_OK;
+}
+
+}  // namespace pbr_host

@@ -70,6 +70,7 @@ class PBRNode(PBRShaderContext):
         self.model_scale_units = model_scale_units
         self.name = name or (model_path if isinstance(model_path, str) else "mesh")
         self.instance_counts: torch.Tensor | None = None
+        self.instance_count_total = 0
 
         if model_path is not None and (not isinstance(model_path, str) or model_path):
             raw = model_path if isinstance(model_path, Mesh) else load_model(model_path)
@@ -236,11 +237,14 @@ class PBRNode(PBRShaderContext):
         """Active instances per scene ([N] int, clamped to [0, ips]); None = all."""
         if counts is None:
             self.instance_counts = None
+            self.instance_count_total = 0
             return
         c = torch.as_tensor(counts).to(device=self.device, dtype=torch.int32).reshape(-1)
         if c.numel() != self.num_scenes:
             raise ValueError(f"counts must have {self.num_scenes} entries, got {c.numel()}")
         self.instance_counts = c.clamp(0, self.instances_per_scene).contiguous()
+        # the batch's instance total sizes the tile path's record pool (one sync here)
+        self.instance_count_total = int(self.instance_counts.sum().item())
 
     # --- misc API of the reference ----------------------------------------
     def set_texture(self, texture=None) -> None:

@@ -267,11 +267,26 @@ class PBRRenderer:
         o.persistent = 1 if self.cfg.persistent else 0
         o.workgroups = 0
         o.kernel = {"auto": _abi.PBR_KERNEL_AUTO, "chunked": _abi.PBR_KERNEL_CHUNKED,
-                    "scene": _abi.PBR_KERNEL_SCENE}[self.cfg.kernel]
+                    "scene": _abi.PBR_KERNEL_SCENE, "tiles": _abi.PBR_KERNEL_TILES}[self.cfg.kernel]
         o.bin_width = int(self.cfg.bin_width)
         o.bins_per_item = int(self.cfg.bins_per_item)
         o.lane_pixels = int(self.cfg.lane_pixels)
+        o.tile_size = int(self.cfg.tile_size)
+        o.slot_budget = self.slot_budget()
         return o
+
+    def slot_budget(self) -> int:
+        """Triangle slots of the whole batch when some node draws per-scene
+        instance counts (sizes the tile path's record pool); 0 otherwise."""
+        nodes = self._drawn_nodes()
+        if not any(n.instance_counts is not None for n in nodes):
+            return 0
+        total = 0
+        for n in nodes:
+            inst = (n.instance_count_total if n.instance_counts is not None
+                    else self.num_scenes * n.instances_per_scene)
+            total += inst * n.mesh.num_triangles
+        return int(total)
 
     def set_poses(self, poses) -> None:
         """PBRNode.set_pose for several nodes, one HIP launch on the GPU.

@@ -79,13 +79,18 @@ def many_cubes(num_scenes: int = 1024, resolution: int = 128, instances: int = 6
 
 def uneven_batch(num_scenes: int = 8192, resolution: int = 84, max_instances: int = 256,
                  seed: int = 0, device: str = "cuda", output_dtype: str = "uint8",
-                 models=("models/box", "models/sphere", "models/cylinder"), **cfg) -> PBRRenderer:
+                 models=("models/box", "models/sphere", "models/cylinder"), uniform: bool = False,
+                 **cfg) -> PBRRenderer:
+    """``uniform=True`` gives every scene the batch's mean instance count (the
+    same draws otherwise): the uniform reference for per-triangle cost."""
     gen = torch.Generator("cpu").manual_seed(seed)
     c = PBRConfig(num_scenes=num_scenes, tile_resolution=(resolution, resolution), device=device,
                   output_dtype=output_dtype, **cfg)
     r = PBRRenderer(c)
     u = torch.rand(num_scenes, generator=gen, dtype=torch.float64)
     total = torch.round(torch.exp(u * math.log(max_instances))).clamp(1, max_instances).long()
+    if uniform:
+        total = torch.full_like(total, int(torch.round(total.double().mean()).item()))
     choice = torch.randint(0, len(models), (num_scenes, max_instances), generator=gen)
     active = torch.arange(max_instances)[None, :] < total[:, None]
     for k, m in enumerate(models):

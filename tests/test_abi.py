@@ -80,5 +80,19 @@ def test_workspace_size():
     d = _abi.pbr_scene_desc()
     o = _abi.pbr_render_opts()
     assert _native.workspace_bytes(d, o) == 0
+    d.num_scenes, d.width, d.height = 3, 32, 32
     o.persistent = 1
-    assert _native.workspace_bytes(d, o) >= 4
+    o.kernel = _abi.PBR_KERNEL_CHUNKED
+    assert _native.workspace_bytes(d, o) == 256  # the queue head
+    # tile path: header, scene offsets, then per slot box + record + colour + 4 shade floats
+    o.kernel = _abi.PBR_KERNEL_TILES
+    d.num_nodes, d.num_meshes = 1, 1
+    d.meshes[0].num_triangles = 100
+    d.nodes[0].instances_per_scene = 2
+    d.nodes[0].flat = 1  # flat meshes keep 4 shade floats per record (smooth: 16)
+    budget = 3 * 200
+    want = 256 + 256 + sum((b + 255) // 256 * 256 for b in (8 * budget, 64 * budget,
+                                                           4 * budget, 16 * budget))
+    assert _native.workspace_bytes(d, o) == want
+    o.slot_budget = 50
+    assert _native.workspace_bytes(d, o) < want

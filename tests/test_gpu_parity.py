@@ -114,21 +114,73 @@ def test_smooth_meshes_uneven_counts(cuda):
 @pytest.mark.parametrize("rows", [8, 16, 24, 64])
 def test_region_rows(cuda, rows):
     r = workloads.many_cubes(num_scenes=16, resolution=64, instances=24, seed=9,
-                             output_dtype="uint8", output_depth=False, region_rows=rows)
+                             output_dtype="uint8", output_depth=False, region_rows=rows,
+                             kernel="chunked")
     check_parity(r, f"region rows {rows}")
 
 
-def test_persistent_queue(cuda):
+@pytest.mark.parametrize("kernel", ["tiles", "chunked"])
+def test_persistent_queue(cuda, kernel):
     r = workloads.uneven_batch(num_scenes=200, resolution=64, max_instances=48, seed=11,
-                               persistent=True)
-    check_parity(r, "persistent work queue")
-    check_parity(r, "persistent work queue (second launch)")
+                               persistent=True, kernel=kernel)
+    check_parity(r, f"persistent work queue ({kernel})")
+    check_parity(r, f"persistent work queue ({kernel}, second launch)")
 
 
-def test_near_plane_and_guard_band_clipping(cuda):
+@pytest.mark.parametrize("ts", [16, 32, 64])
+@pytest.mark.parametrize("res", [(84, 84), (128, 72), (50, 30)])
+def test_tile_sizes(cuda, ts, res):
+    gen = torch.Generator("cpu").manual_seed(13)
+    c = PBRConfig(num_scenes=20, tile_resolution=res, device="cuda", kernel="tiles",
+                  tile_size=ts, output_depth=True)
+    r = PBRRenderer(c)
+    box = r.add_node("models/box", instances_per_scene=20)
+    box.set_transforms(workloads.random_transforms(20 * 20, gen))
+    box.set_colors(torch.rand(20 * 20, 4, generator=gen))
+    sph = r.add_node("models/sphere", instances_per_scene=6)
+    sph.set_transforms(workloads.random_transforms(20 * 6, gen, scale_range=(0.3, 2.5)))
+    sph.set_colors(torch.rand(20 * 6, 4, generator=gen))
+    sph.set_instance_counts(torch.randint(0, 7, (20,), generator=gen))
+    r.add_camera()
+    r._pbr_cam.set_positions_and_lookat(torch.tensor([0.0, -7.0, 2.0]), torch.zeros(3))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, f"tiles {ts} at {res}")
+
+
+def test_tiles_uneven_batch_full_size_slice(cuda):
+    """The uneven-batch workload's scene mix at its real resolution."""
+    r = workloads.uneven_batch(num_scenes=64, resolution=84, max_instances=256, seed=0,
+                               kernel="tiles", persistent=True)
+    check_parity(r, "uneven 84x84 up to 256 instances")
+
+
+@pytest.mark.parametrize("kernel", ["tiles", "chunked"])
+def test_smooth_geometry_through_the_near_plane(cuda, kernel):
+    """Spheres and boxes straddling the near plane: clipped smooth sub-triangles."""
+    n = 12
+    c = PBRConfig(num_scenes=n, tile_resolution=(48, 48), device="cuda", kernel=kernel,
+                  output_dtype="float32", output_depth=True)
+    r = PBRRenderer(c)
+    sph = r.add_node("models/sphere", instances_per_scene=3)
+    gen = torch.Generator("cpu").manual_seed(4)
+    M = workloads.random_transforms(n * 3, gen, pos_range=1.2, scale_range=(2.0, 4.0))
+    sph.set_transforms(M)
+    sph.set_colors(torch.rand(n * 3, 4, generator=gen))
+    r.add_camera(fov_y_deg=80.0, z_near=0.8, z_far=40.0)
+    eye = torch.zeros(n, 3)
+    eye[:, 1] = -1.5 + 0.1 * torch.arange(n)
+    r._pbr_cam.set_positions_and_lookat(eye, torch.tensor([0.0, 5.0, 0.0]))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, f"smooth near-plane clipping ({kernel})")
+
+
+@pytest.mark.parametrize("kernel", ["tiles", "chunked"])
+def test_near_plane_and_guard_band_clipping(cuda, kernel):
     """Geometry that crosses the near plane and extends far past the view."""
     n = 16
-    c = PBRConfig(num_scenes=n, tile_resolution=(64, 64), device="cuda")
+    c = PBRConfig(num_scenes=n, tile_resolution=(64, 64), device="cuda", kernel=kernel)
     r = PBRRenderer(c)
     ground = r.add_node("models/quad", instances_per_scene=1, shared_across_scenes=True,
                         model_scale=(4000.0, 4000.0, 1.0), model_pivot_relative_point=(0.5, 0.5, 0.0))
@@ -145,7 +197,7 @@ def test_near_plane_and_guard_band_clipping(cuda):
     r._pbr_cam.set_positions_and_lookat(eye, torch.tensor([0.0, 10.0, 0.5]))
     r.add_light()
     r.setup_environment()
-    check_parity(r, "near/far/guard clipping")
+    check_parity(r, f"near/far/guard clipping ({kernel})")
 
 
 def test_empty_and_zero_instances(cuda):
@@ -247,7 +299,7 @@ def test_scene_kernel_heavy_clipping_multi_pass(cuda):
     check_parity(r, "scene kernel clipping")
 
 
-@pytest.mark.parametrize("kernel", ["scene", "chunked"])
+@pytest.mark.parametrize("kernel", ["scene", "chunked", "tiles"])
 def test_cartpole_both_kernels(cuda, kernel):
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=512, device="cuda", kernel=kernel))
     r._step(ground_states(512, 5).cuda())
