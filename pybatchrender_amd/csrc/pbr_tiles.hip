@@ -27,6 +27,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "../../include/pbr_abi.h"
 #include "pbr_device.h"
 #include "pbr_host.h"
@@ -35,13 +37,16 @@ using namespace pbr_host;
 
 namespace {
 
-constexpr int kTileBlock = 256;
-constexpr int kQCap = 512;       // entries per LDS queue
+constexpr int kTileBlock = 512;
+constexpr int kPerThread = 2;                        // boxes one thread tests per round
+constexpr int kRound = kTileBlock * kPerThread;      // boxes per round
+constexpr int kQCap = 2 * kRound;                    // entries per LDS queue
+constexpr int kAutoTilePixels = 8192;                // auto tiles: 64 KiB of pixel state
 constexpr int kSmallArea = 48;   // box-in-tile pixels one lane rasterises alone
 constexpr uint32_t kDepthOne = 0x3f800000u;
 
 struct TileFrame {
-  int W, H, ts, ntx, tiles;  // tile edge, tiles per row, tiles per scene
+  int W, H, tw, th, ntx, tiles;  // tile width / height, tiles per row, tiles per scene
   float guard;
   uint32_t max_slots;        // slots of the largest scene
   int uniform;               // 1: no node has per-scene counts
@@ -139,23 +144,59 @@ __device__ __forceinline__ bool clipped_box(const CVert cv[3], const TileFrame& 
   return i0 <= i1;
 }
 
-// Block b handles slots [256 (b % chunks), +256) of scene b / chunks.
+// Pool slots in use: every slot of a uniform batch, else the prefix total.
+__device__ __forceinline__ uint32_t pool_total(const pbr_scene_desc& D, const TileFrame& F,
+                                               const TilePool& P) {
+  const uint64_t t = F.uniform ? (uint64_t)D.num_scenes * F.max_slots : P.scene_off[D.num_scenes];
+  return (uint32_t)min(t, (uint64_t)F.budget);
+}
+
+// Scene s and its slot of pool slot g (binary search of the scene offsets
+// when instance counts differ per scene).
+__device__ __forceinline__ void pool_locate(const pbr_scene_desc& D, const TileFrame& F,
+                                            const TilePool& P, uint32_t g, int& s,
+                                            uint32_t& slot) {
+  if (F.uniform) {
+    s = (int)(g / F.max_slots);
+    slot = g - (uint32_t)s * F.max_slots;
+    return;
+  }
+  int lo = 0, hi = D.num_scenes;  // largest s with scene_off[s] <= g
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (P.scene_off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  s = lo;
+  slot = g - P.scene_off[lo];
+}
+
+// Node of a scene slot and the slot's index within the node's instances.
+__device__ __forceinline__ void locate_node(const pbr_scene_desc& D, int s, uint32_t slot, int& j,
+                                            uint32_t& local) {
+  uint32_t acc = 0;
+  j = 0;
+  local = slot;
+  for (int k = 0; k < D.num_nodes; ++k) {
+    const pbr_node& nd = D.nodes[k];
+    int cnt = nd.instances_per_scene;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
+    const uint32_t n = (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
+    if (slot < acc + n) {
+      j = k;
+      local = slot - acc;
+      return;
+    }
+    acc += n;
+  }
+}
+
+// Setup of pool slot g (scene slot `slot`): box, record, colours.
 template <bool SMOOTH>
-__global__ __launch_bounds__(256) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
-                                                         TilePool P, Light lt, uint32_t chunks) {
-  __shared__ uint32_t base[PBR_MAX_NODES + 1];
-  const int s = (int)(blockIdx.x / chunks);
-  const uint32_t slot = (blockIdx.x % chunks) * 256u + threadIdx.x;
-  if (threadIdx.x == 0) node_table(D, s, base);
-  __syncthreads();
-  uint32_t off, n;
-  scene_range(F, P, s, off, n);
-  if (slot >= n) return;
-  const uint32_t g = off + slot;
-  CVert cv[3];
-  float rgba[4];
-  bool flat;
-  fetch_triangle(D, s, base, slot, cv, rgba, flat);
+__device__ __forceinline__ void tile_setup_one(const TileFrame& F, const TilePool& P,
+                                               const Light& lt, uint32_t g, uint32_t slot,
+                                               const CVert cv[3], const float rgba[4],
+                                               bool flat) {
   float flat_rgba[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   if (!SMOOTH || flat) shade(cv[0].n, rgba, lt, flat_rgba);
   bool all_in, all_out;
@@ -192,6 +233,26 @@ __global__ __launch_bounds__(256) void tile_setup_kernel(pbr_scene_desc D, TileF
   }
   P.box[g] = bb;
   if (!F.float_out) P.col8[g] = pack_rgba8(flat_rgba);
+}
+
+// Grid-stride over the pool: thread -> pool slot g -> (scene, slot).
+template <bool SMOOTH>
+__global__ __launch_bounds__(256) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
+                                                         TilePool P, Light lt) {
+  const uint32_t total = pool_total(D, F, P);
+  for (uint32_t g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
+    int s;
+    uint32_t slot;
+    pool_locate(D, F, P, g, s, slot);
+    int j;
+    uint32_t local;
+    locate_node(D, s, slot, j, local);
+    CVert cv[3];
+    float rgba[4];
+    bool flat;
+    fetch_node_triangle(D, s, j, local, cv, rgba, flat);
+    tile_setup_one<SMOOTH>(F, P, lt, g, slot, cv, rgba, flat);
+  }
 }
 
 struct TileCtx {
@@ -322,20 +383,23 @@ __device__ __forceinline__ void load_grec(const GRec* g, GRec& r) {
   for (int q = 0; q < 4; ++q) dst[q] = src[q];
 }
 
-template <bool SMOOTH, int CMODE, int TS>
+// Dynamic LDS: pixel keys [th][tw], then the small and large queues.
+template <bool SMOOTH, int CMODE>
 __global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc D,
                                                                  pbr_frame_out O, TileFrame F,
                                                                  TilePool P, Light lt, float4 bg,
                                                                  uint32_t total_items,
                                                                  int persistent) {
-  __shared__ unsigned long long pix[TS * TS];
-  __shared__ uint32_t qs[kQCap], qb[kQCap];
+  extern __shared__ unsigned long long pix[];
+  uint32_t* qs = reinterpret_cast<uint32_t*>(pix + F.tw * F.th);
+  uint32_t* qb = qs + kQCap;
   __shared__ uint32_t ns, nb, item_sh;
   __shared__ uint32_t base[PBR_MAX_NODES + 1];
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int W = F.W, H = F.H;
   const float bgc[4] = {bg.x, bg.y, bg.z, bg.w};
+  const int tpx = F.tw * F.th;
 
   uint32_t item = blockIdx.x;
   if (persistent) {
@@ -345,21 +409,20 @@ __global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc 
   }
   while (item < total_items) {
     TileCtx t;
-    t.ps = TS;
+    t.ps = F.tw;
     t.s = (int)(item / (uint32_t)F.tiles);
     const int tt = (int)(item % (uint32_t)F.tiles);
-    t.x0 = (tt % F.ntx) * TS;
-    t.yo0 = (tt / F.ntx) * TS;
-    t.w = min(TS, W - t.x0);
-    t.h = min(TS, H - t.yo0);
+    t.x0 = (tt % F.ntx) * F.tw;
+    t.yo0 = (tt / F.ntx) * F.th;
+    t.w = min(F.tw, W - t.x0);
+    t.h = min(F.th, H - t.yo0);
     t.ti0 = t.x0;
     t.ti1 = t.x0 + t.w - 1;
     t.tj1 = H - 1 - t.yo0;
     t.tj0 = t.tj1 - (t.h - 1);
     uint32_t n;
     scene_range(F, P, t.s, t.off, n);
-    for (int p = tid; p < TS * TS; p += kTileBlock)
-      pix[p] = (unsigned long long)kDepthOne << 32;
+    for (int p = tid; p < tpx; p += kTileBlock) pix[p] = (unsigned long long)kDepthOne << 32;
     if (tid == 0) {
       ns = 0u;
       nb = 0u;
@@ -367,35 +430,39 @@ __global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc 
     }
     __syncthreads();
 
-    for (uint32_t r0 = 0; r0 < n; r0 += kTileBlock) {
-      const uint32_t slot = r0 + (uint32_t)tid;
-      bool small = false, big = false;
-      if (slot < n) {
-        const short4 b = P.box[t.off + slot];
-        const int a0 = max((int)b.x, t.ti0), a1 = min((int)b.y, t.ti1);
-        const int c0 = max((int)b.z, t.tj0), c1 = min((int)b.w, t.tj1);
-        if (a0 <= a1 && c0 <= c1) {
-          small = (a1 - a0 + 1) * (c1 - c0 + 1) <= kSmallArea;
-          big = !small;
-        }
-      }
-      // wave-aggregated enqueue (ballot + prefix within the wave)
+    for (uint32_t r0 = 0; r0 < n; r0 += kRound) {
+      // test kPerThread boxes per thread, compact the survivors (ballot +
+      // prefix within the wave, one LDS atomic per wave and queue)
       const uint64_t below = (1ull << lane) - 1ull;
-      const uint64_t ms = __ballot(small), mb = __ballot(big);
-      uint32_t bs = 0, bbq = 0;
-      if (lane == 0) {
-        if (ms) bs = atomicAdd(&ns, (uint32_t)__popcll(ms));
-        if (mb) bbq = atomicAdd(&nb, (uint32_t)__popcll(mb));
+#pragma unroll
+      for (int k = 0; k < kPerThread; ++k) {
+        const uint32_t slot = r0 + (uint32_t)(k * kTileBlock + tid);
+        bool small = false, big = false;
+        if (slot < n) {
+          const short4 b = P.box[t.off + slot];
+          const int a0 = max((int)b.x, t.ti0), a1 = min((int)b.y, t.ti1);
+          const int c0 = max((int)b.z, t.tj0), c1 = min((int)b.w, t.tj1);
+          if (a0 <= a1 && c0 <= c1) {
+            small = (a1 - a0 + 1) * (c1 - c0 + 1) <= kSmallArea;
+            big = !small;
+          }
+        }
+        const uint64_t ms = __ballot(small), mb = __ballot(big);
+        uint32_t bs = 0, bbq = 0;
+        if (lane == 0) {
+          if (ms) bs = atomicAdd(&ns, (uint32_t)__popcll(ms));
+          if (mb) bbq = atomicAdd(&nb, (uint32_t)__popcll(mb));
+        }
+        bs = __shfl(bs, 0);
+        bbq = __shfl(bbq, 0);
+        if (small) qs[bs + (uint32_t)__popcll(ms & below)] = slot;
+        if (big) qb[bbq + (uint32_t)__popcll(mb & below)] = slot;
       }
-      bs = __shfl(bs, 0);
-      bbq = __shfl(bbq, 0);
-      if (small) qs[bs + (uint32_t)__popcll(ms & below)] = slot;
-      if (big) qb[bbq + (uint32_t)__popcll(mb & below)] = slot;
       __syncthreads();
-      const bool last = r0 + kTileBlock >= n;
+      const bool last = r0 + kRound >= n;
       const uint32_t cs = ns, cb = nb;
-      const bool flush_s = cs > (uint32_t)(kQCap - kTileBlock) || (last && cs > 0);
-      const bool flush_b = cb > (uint32_t)(kQCap - kTileBlock) || (last && cb > 0);
+      const bool flush_s = cs > (uint32_t)(kQCap - kRound) || (last && cs > 0);
+      const bool flush_b = cb > (uint32_t)(kQCap - kRound) || (last && cb > 0);
       if (flush_s) {
         for (uint32_t k = (uint32_t)tid; k < cs; k += kTileBlock) {
           const uint32_t sl = qs[k];
@@ -433,7 +500,7 @@ __global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc 
     const size_t plane = (size_t)H * (size_t)W;
     for (int p = tid; p < t.w * t.h; p += kTileBlock) {
       const int ly = p / t.w, lx = p - ly * t.w;
-      const unsigned long long v = pix[ly * TS + lx];
+      const unsigned long long v = pix[ly * F.tw + lx];
       const uint32_t zb = (uint32_t)(v >> 32);
       const uint32_t rank = (uint32_t)v;
       const bool covered = zb < kDepthOne;
@@ -531,14 +598,15 @@ struct TilePlan {
   bool counts;
   bool smooth;
   size_t off_scene, off_box, off_recs, off_col8, off_shades, bytes;
+  size_t lds;  // dynamic LDS of the raster kernel
 };
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 int tiles_plan(const pbr_scene_desc* D, const pbr_render_opts* opts, TilePlan& T) {
   memset(&T, 0, sizeof(T));
-  const int ts = (opts && opts->tile_size > 0) ? opts->tile_size : 32;
-  if (ts != 16 && ts != 32 && ts != 64)
+  const int ts = (opts && opts->tile_size > 0) ? opts->tile_size : 0;
+  if (ts != 0 && ts != 16 && ts != 32 && ts != 64)
     return fail(PBR_ERR_INVALID, "tile_size %d is not 16, 32 or 64", ts);
   const uint64_t ms = max_slots(D);
   if (ms >= kMaxTileSlots)
@@ -556,9 +624,15 @@ int tiles_plan(const pbr_scene_desc* D, const pbr_render_opts* opts, TilePlan& T
   TileFrame& F = T.F;
   F.W = D->width;
   F.H = D->height;
-  F.ts = ts;
-  F.ntx = (D->width + ts - 1) / ts;
-  F.tiles = F.ntx * ((D->height + ts - 1) / ts);
+  if (ts > 0) {
+    F.tw = F.th = ts;
+  } else {  // full-width bands of at most 64 KiB of pixel keys
+    F.tw = D->width;
+    F.th = std::min(D->height, std::max(1, kAutoTilePixels / D->width));
+  }
+  F.ntx = (D->width + F.tw - 1) / F.tw;
+  F.tiles = F.ntx * ((D->height + F.th - 1) / F.th);
+  T.lds = sizeof(unsigned long long) * (size_t)F.tw * F.th + 2 * sizeof(uint32_t) * kQCap;
   F.guard = 65536.0f / (float)(D->width > D->height ? D->width : D->height);
   F.max_slots = (uint32_t)ms;
   F.uniform = T.counts ? 0 : 1;
@@ -582,21 +656,15 @@ int tiles_plan(const pbr_scene_desc* D, const pbr_render_opts* opts, TilePlan& T
 template <bool SMOOTH, int CMODE>
 hipError_t launch_tile_raster(const pbr_scene_desc* D, const pbr_frame_out* O, const TileFrame& F,
                               const TilePool& P, const Light& lt, float4 bg, uint32_t items,
-                              int grid, int persistent, hipStream_t st) {
-  switch (F.ts) {
-    case 16:
-      hipLaunchKernelGGL((tile_raster_kernel<SMOOTH, CMODE, 16>), dim3(grid), dim3(kTileBlock), 0,
-                         st, *D, *O, F, P, lt, bg, items, persistent);
-      break;
-    case 64:
-      hipLaunchKernelGGL((tile_raster_kernel<SMOOTH, CMODE, 64>), dim3(grid), dim3(kTileBlock), 0,
-                         st, *D, *O, F, P, lt, bg, items, persistent);
-      break;
-    default:
-      hipLaunchKernelGGL((tile_raster_kernel<SMOOTH, CMODE, 32>), dim3(grid), dim3(kTileBlock), 0,
-                         st, *D, *O, F, P, lt, bg, items, persistent);
-      break;
+                              int grid, int persistent, size_t lds, hipStream_t st) {
+  auto k = tile_raster_kernel<SMOOTH, CMODE>;
+  if (lds > 64 * 1024) {
+    const hipError_t e =
+        hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
   }
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kTileBlock), lds, st, *D, *O, F, P, lt, bg, items,
+                     persistent);
   return hipGetLastError();
 }
 
@@ -639,16 +707,15 @@ int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rend
     e = hipGetLastError();
     if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile prep launch failed: %s", hipGetErrorString(e));
   }
-  const uint32_t chunks = (F.max_slots + 255) / 256;
-  const uint64_t sblocks = (uint64_t)chunks * (uint64_t)D->num_scenes;
-  if (sblocks >= 0xffffffffull) return fail(PBR_ERR_UNSUPPORTED, "too many setup blocks");
+  // grid-stride setup: enough blocks to fill the chip, never more than the pool
+  const uint64_t need = ((uint64_t)F.budget + 255) / 256;
+  const uint64_t fill = (uint64_t)device_cus() * 16;
+  const uint32_t sblocks = (uint32_t)std::min(need, fill);
   if (sblocks > 0) {
     if (T.smooth)
-      hipLaunchKernelGGL(tile_setup_kernel<true>, dim3((uint32_t)sblocks), dim3(256), 0, st, *D,
-                         F, P, lt, chunks);
+      hipLaunchKernelGGL(tile_setup_kernel<true>, dim3(sblocks), dim3(256), 0, st, *D, F, P, lt);
     else
-      hipLaunchKernelGGL(tile_setup_kernel<false>, dim3((uint32_t)sblocks), dim3(256), 0, st, *D,
-                         F, P, lt, chunks);
+      hipLaunchKernelGGL(tile_setup_kernel<false>, dim3(sblocks), dim3(256), 0, st, *D, F, P, lt);
     e = hipGetLastError();
     if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile setup launch failed: %s", hipGetErrorString(e));
   }
@@ -663,12 +730,15 @@ int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rend
   const float4 bg = make_float4(D->background[0], D->background[1], D->background[2],
                                 D->background[3]);
   const int cmode = O->dtype == PBR_DTYPE_U8 ? 0 : 1;
+  const size_t lds = T.lds;
   if (T.smooth)
-    e = cmode == 0 ? launch_tile_raster<true, 0>(D, O, F, P, lt, bg, items, grid, persistent, st)
-                   : launch_tile_raster<true, 1>(D, O, F, P, lt, bg, items, grid, persistent, st);
+    e = cmode == 0
+            ? launch_tile_raster<true, 0>(D, O, F, P, lt, bg, items, grid, persistent, lds, st)
+            : launch_tile_raster<true, 1>(D, O, F, P, lt, bg, items, grid, persistent, lds, st);
   else
-    e = cmode == 0 ? launch_tile_raster<false, 0>(D, O, F, P, lt, bg, items, grid, persistent, st)
-                   : launch_tile_raster<false, 1>(D, O, F, P, lt, bg, items, grid, persistent, st);
+    e = cmode == 0
+            ? launch_tile_raster<false, 0>(D, O, F, P, lt, bg, items, grid, persistent, lds, st)
+            : launch_tile_raster<false, 1>(D, O, F, P, lt, bg, items, grid, persistent, lds, st);
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile raster launch failed: %s", hipGetErrorString(e));
   return PBR_OK;
 }
