@@ -265,7 +265,7 @@ class PBRRenderer:
         o = _abi.pbr_render_opts()
         o.region_rows = int(self.cfg.region_rows)
         o.persistent = 1 if self.cfg.persistent else 0
-        o.workgroups = 0
+        o.workgroups = int(getattr(self, "workgroups", 0))  # persistent launch size (autotune)
         o.kernel = {"auto": _abi.PBR_KERNEL_AUTO, "chunked": _abi.PBR_KERNEL_CHUNKED,
                     "scene": _abi.PBR_KERNEL_SCENE, "tiles": _abi.PBR_KERNEL_TILES}[self.cfg.kernel]
         o.bin_width = int(self.cfg.bin_width)

@@ -361,3 +361,15 @@ def test_batched_poses_match_single(cuda):
     b.set_pose(y=1.0, h=0.3, s=2.0)
     torch.cuda.synchronize()
     assert torch.equal(ta, a.transforms_b44) and torch.equal(tb, b.transforms_b44)
+
+
+def test_autotune_picks_a_shape_that_matches_the_oracle(cuda, tmp_path):
+    from pybatchrender_amd import autotune
+    r = workloads.uneven_batch(num_scenes=48, resolution=84, max_instances=32, seed=21)
+    cache = tmp_path / "tune.json"
+    rep = autotune.autotune(r, launches=2, cache_path=cache)
+    assert not rep["cached"] and len(rep["results"]) >= 16
+    print(f"[autotune] best {rep['best']} {rep['us']} us", flush=True)
+    check_parity(r, f"autotuned {rep['best']}")
+    rep2 = autotune.autotune(r, cache_path=cache)
+    assert rep2["cached"] and rep2["best"] == rep["best"]

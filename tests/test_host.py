@@ -217,3 +217,18 @@ def test_cartpole_dynamics_closed_form():
     xacc = temp - mp * l * tacc * math.cos(th) / (mc + mp)
     want = [x + tau * xd, xd + tau * xacc, th + tau * thd, thd + tau * tacc]
     assert np.allclose(nxt[0].numpy(), want, atol=1e-6)
+
+
+def test_autotune_candidates_and_signature():
+    from pybatchrender_amd import autotune
+    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=8, device="cpu"))
+    cands = autotune.candidates(r)
+    assert any(c["kernel"] == "scene" for c in cands)  # 26 slots: the scene kernel qualifies
+    assert {c.get("tile_size") for c in cands if c["kernel"] == "tiles"} == {0, 16, 32, 64}
+    sig = autotune.signature(r)
+    assert '"N": 8' in sig and '"W": 64' in sig
+    autotune._apply(r, {"kernel": "tiles", "tile_size": 32, "persistent": False})
+    o = r.render_opts()
+    assert o.kernel == 3 and o.tile_size == 32 and o.persistent == 0
