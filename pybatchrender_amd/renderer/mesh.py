@@ -13,6 +13,9 @@ not available here, so the built-ins are generated:
                      smooth normals, 224 triangles
     models/cylinder  radius 0.5, height 1 along z, 32 segments: smooth
                      sides (64 triangles) + flat caps (64), 128 triangles
+    models/cone      radius 0.5, height 1 along z, 32 segments, 64 triangles
+
+Paths ending in .gltf / .glb / .egg are read from disk by ``model_io``.
 
 Baking runs in float64 on the host and is stored as float32 (one upload at
 ``setup_environment``).
@@ -147,7 +150,40 @@ def cylinder(segments: int = 32, radius: float = 0.5, height: float = 1.0) -> Me
     return Mesh(np.array(pos), np.array(nrm), np.array(tris), "cylinder")
 
 
+def cone(segments: int = 32, radius: float = 0.5, height: float = 1.0) -> Mesh:
+    """Cone along +z: base disc at z = -height/2, apex at +height/2; side
+    normals of the slanted surface per ring vertex, flat base."""
+    pos, nrm, tris = [], [], []
+    h0, h1 = -0.5 * height, 0.5 * height
+    slope = radius / height
+    k = 1.0 / math.sqrt(1.0 + slope * slope)
+    for s in range(segments):  # one apex per segment so each side face has its own normal
+        ph0 = 2 * math.pi * s / segments
+        ph1 = 2 * math.pi * (s + 1) / segments
+        phm = 0.5 * (ph0 + ph1)
+        base = len(pos)
+        pos += [(radius * math.cos(ph0), radius * math.sin(ph0), h0),
+                (radius * math.cos(ph1), radius * math.sin(ph1), h0), (0.0, 0.0, h1)]
+        nrm += [(k * math.cos(ph0), k * math.sin(ph0), k * slope),
+                (k * math.cos(ph1), k * math.sin(ph1), k * slope),
+                (k * math.cos(phm), k * math.sin(phm), k * slope)]
+        tris.append((base, base + 1, base + 2))
+    centre = len(pos)
+    pos.append((0.0, 0.0, h0))
+    nrm.append((0.0, 0.0, -1.0))
+    first = len(pos)
+    for s in range(segments):
+        ph = 2 * math.pi * s / segments
+        pos.append((radius * math.cos(ph), radius * math.sin(ph), h0))
+        nrm.append((0.0, 0.0, -1.0))
+    for s in range(segments):
+        tris.append((centre, first + (s + 1) % segments, first + s))
+    return Mesh(np.array(pos), np.array(nrm), np.array(tris), "cone")
+
+
 _BUILTINS = {
+    "models/cone": cone,
+    "cone": cone,
     "models/box": box,
     "box": box,
     "models/quad": quad,
@@ -159,17 +195,37 @@ _BUILTINS = {
 }
 
 
+MODEL_PATH: list[str] = []  # extra directories searched for model files
+
+
 def load_model(path: str) -> Mesh:
-    """Mesh for a model path.  Built-in names resolve to procedural meshes."""
+    """Mesh for a model path (node.py:61-62 loadModel).
+
+    An existing .gltf / .glb / .egg file (as given, or under a directory of
+    ``MODEL_PATH`` or the ``PBR_MODEL_PATH`` environment variable) is read by
+    ``model_io``; otherwise built-in names resolve to procedural meshes."""
+    import os
+    from pathlib import Path
+
+    from . import model_io
+
     key = str(path)
+    if Path(key).suffix.lower() in (".gltf", ".glb", ".egg"):
+        dirs = [""] + MODEL_PATH + [d for d in os.environ.get("PBR_MODEL_PATH", "").split(
+            os.pathsep) if d]
+        for d in dirs:
+            cand = Path(d) / key if d else Path(key)
+            if cand.is_file():
+                return model_io.load_file(cand)
     for suffix in (".egg", ".bam", ".egg.pz"):
         if key.endswith(suffix) and key[: -len(suffix)] in _BUILTINS:
             key = key[: -len(suffix)]
     if key in _BUILTINS:
         return _BUILTINS[key]()
     raise FileNotFoundError(
-        f"model '{path}' is not a built-in mesh ({sorted(set(_BUILTINS))}); "
-        "pass a Mesh instance instead"
+        f"model '{path}' is neither a readable .gltf/.glb/.egg file nor a built-in mesh "
+        f"({sorted(set(_BUILTINS))}); pass a Mesh instance or add its directory to "
+        "MODEL_PATH / PBR_MODEL_PATH"
     )
 
 

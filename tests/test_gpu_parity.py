@@ -373,3 +373,26 @@ def test_autotune_picks_a_shape_that_matches_the_oracle(cuda, tmp_path):
     check_parity(r, f"autotuned {rep['best']}")
     rep2 = autotune.autotune(r, cache_path=cache)
     assert rep2["cached"] and rep2["best"] == rep["best"]
+
+
+def test_meshes_loaded_from_files(cuda, tmp_path):
+    """Nodes built from an egg and a glTF file on disk render like any mesh."""
+    from pybatchrender_amd.renderer import mesh as meshmod
+    from tests.test_models import _quad_gltf, write_egg
+    egg = write_egg(meshmod.cone(16), tmp_path / "cone.egg")
+    gltf = _quad_gltf(tmp_path, double_sided=True)
+    n = 10
+    c = PBRConfig(num_scenes=n, tile_resolution=(48, 48), device="cuda")
+    r = PBRRenderer(c)
+    gen = torch.Generator("cpu").manual_seed(31)
+    a = r.add_node(str(egg), instances_per_scene=3, model_scale=1.5)
+    a.set_transforms(workloads.random_transforms(n * 3, gen, pos_range=1.5, scale_range=(0.8, 1.6)))
+    a.set_colors(torch.rand(n * 3, 4, generator=gen))
+    b = r.add_node(str(gltf), instances_per_scene=2, model_pivot_relative_point=(0.5, 0.5, 0.5))
+    b.set_transforms(workloads.random_transforms(n * 2, gen, pos_range=1.0, scale_range=(1.0, 2.0)))
+    b.set_colors(torch.rand(n * 2, 4, generator=gen))
+    r.add_camera()
+    r._pbr_cam.set_positions_and_lookat(torch.tensor([0.0, -6.0, 2.0]), torch.zeros(3))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, "meshes from egg / glTF files")
