@@ -13,7 +13,8 @@ import torch
 
 try:  # pragma: no cover - exercised only where torchrl is installed
     from tensordict import TensorDict  # type: ignore
-    from torchrl.data.tensor_specs import Categorical, Composite, Unbounded  # type: ignore
+    from torchrl.data.tensor_specs import (Bounded, Categorical, Composite,  # type: ignore
+                                           Unbounded)
     from torchrl.envs import EnvBase  # type: ignore
 
     HAVE_TORCHRL = True
@@ -87,6 +88,15 @@ except Exception:  # torchrl / tensordict not installed
         def rand(self):
             return torch.randint(0, self.n, self.shape, dtype=self.dtype, device=self.device)
 
+    class Bounded(_Spec):
+        def __init__(self, low, high, shape, dtype=torch.float32, device=None):
+            super().__init__(shape, dtype, device)
+            self.low, self.high = float(low), float(high)
+
+        def rand(self):
+            u = torch.rand(self.shape, dtype=self.dtype, device=self.device)
+            return self.low + (self.high - self.low) * u
+
     class Composite(dict):
         def __init__(self, shape=None, **fields):
             super().__init__(fields)
@@ -123,4 +133,5 @@ except Exception:  # torchrl / tensordict not installed
         def close(self):
             pass
 
-__all__ = ["TensorDict", "EnvBase", "Composite", "Unbounded", "Categorical", "HAVE_TORCHRL"]
+__all__ = ["TensorDict", "EnvBase", "Composite", "Unbounded", "Categorical", "Bounded",
+           "HAVE_TORCHRL"]

@@ -52,6 +52,8 @@ def _register_builtins() -> None:
                            CartPoleRenderer)
     register("CartPole-v0", CartPoleEnv, CartPoleRenderer, CartPoleConfig)
     register("CartPoleGround-v0", CartPoleEnv, CartPoleGroundRenderer, CartPoleConfig)
+    from .steering import SteeringConfig, SteeringEnv, SteeringRenderer
+    register("Steering-v0", SteeringEnv, SteeringRenderer, SteeringConfig)
 
 
 def _register_from_entry_points() -> None:
