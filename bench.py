@@ -19,6 +19,12 @@ Printed JSON line (one, from this single process):
   cpu_baseline the CPU oracle (a C restatement of the same render path)
                rendering a 256-scene sample of the same workload on all host
                cores
+  fwd_bwd      the same workload with float32 output: plain forward vs
+               forward with the visibility buffer + pbr_render_backward
+               (colour, light and vertex gradients), C-ABI launches timed
+               with HIP events, and the autograd step through diff.render
+  env_loop     16384-scene on-device CartPole loop (pbr_cartpole_step +
+               render, bf16 RGB + depth), one hipGraph replay per step
   env_step     CartPole-v0 (reference scene) env.step at the same size:
                dynamics + auto-reset + render, scene-frames/s
 """
@@ -355,6 +361,8 @@ def main() -> int:
     ap.add_argument("--loop-steps", type=int, default=1000)
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the back-to-back render launches (for counter profiles)")
+    ap.add_argument("--bwd-only", action="store_true",
+                    help="only the fwd+bwd leg (for kernel profiles of the backward)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="with --kernel-only: also copy 256 MiB device-to-device 5 times")
     args = ap.parse_args()
@@ -379,6 +387,10 @@ def main() -> int:
         print(json.dumps({"kernel_us": kus, "bytes_per_launch": algorithmic_bytes(r)}), flush=True)
         if args.copy_calibration:
             copy_calibration()
+        return 0
+    if args.bwd_only:
+        log("[bench] fwd+bwd leg only")
+        print(json.dumps(fwd_bwd_leg(n, states[0])), flush=True)
         return 0
     log(f"[bench] eager: warmup {args.warmup} steps, N={n}")
     for i in range(args.warmup):

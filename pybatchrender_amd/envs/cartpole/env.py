@@ -90,10 +90,9 @@ class CartPoleEnv(PBREnv):
             "done": torch.zeros(*bs, 1, dtype=torch.bool, device=self.device),
         }
         if self.render:
-            try:
-                fields["pixels"] = self.render_pixels(state)
-            except Exception:  # reference behaviour: no pixels on render failure (:171-176)
-                pass
+            # the reference drops `pixels` when rendering fails (:171-176);
+            # here a render error propagates
+            fields["pixels"] = self.render_pixels(state)
         return TensorDict(fields, batch_size=self.batch_size)
 
     @torch.no_grad()
@@ -111,12 +110,7 @@ class CartPoleEnv(PBREnv):
         next_obs_raw = self._dynamics(obs, action)
         done = self._termination(next_obs_raw, step_count)
         reward = self._reward(next_obs_raw, done)
-        pixels = None
-        if self.render:
-            try:
-                pixels = self.render_pixels(next_obs_raw)
-            except Exception:
-                pixels = None
+        pixels = self.render_pixels(next_obs_raw) if self.render else None
         if self.auto_reset:
             reset_state = self._sample_initial_state(next_obs_raw.shape[:-1])
             next_obs = torch.where(done, reset_state, next_obs_raw)

@@ -194,17 +194,15 @@ class SteeringEnv(PBREnv):
                   "step_count": torch.zeros(*bs, dtype=torch.long, device=self.device),
                   "done": torch.zeros(*bs, 1, dtype=torch.bool, device=self.device)}
         if self.render_enabled:
-            try:
-                fields["pixels"] = self.render_pixels(fields["observation"])
-            except Exception:  # reference behaviour: no pixels on render failure
-                pass
+            fields["pixels"] = self.render_pixels(fields["observation"])
         return TensorDict(fields, batch_size=self.batch_size)
 
     @torch.no_grad()
     def _step(self, tensordict: TensorDict) -> TensorDict:
         """env.py:416-456.  Auto-reset keeps the reference's behaviour: done
         envs restart their step count but keep their state (the reference's
-        partial reset is a TODO there)."""
+        partial reset is a TODO there).  A render error propagates (the
+        reference drops the pixels key and carries on)."""
         obs = tensordict.get("observation", None)
         if obs is None:
             td0 = self._reset()
@@ -218,12 +216,7 @@ class SteeringEnv(PBREnv):
         next_obs = self._get_observation()
         done = self._termination(crashes, reached_end, step_count)
         reward = self._reward(crashes, done)
-        pixels = None
-        if self.render_enabled:
-            try:
-                pixels = self.render_pixels(next_obs)
-            except Exception:
-                pixels = None
+        pixels = self.render_pixels(next_obs) if self.render_enabled else None
         if self.auto_reset:
             next_step_count = torch.where(done.squeeze(-1), torch.zeros_like(step_count),
                                           step_count + 1)

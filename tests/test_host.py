@@ -198,12 +198,22 @@ def test_cartpole_env_without_pixels_on_cpu():
     assert td["reward"].shape == (32, 1) and "pixels" not in td
 
 
-def test_cartpole_env_drops_pixels_when_render_fails_on_cpu():
-    """Reference behaviour (cartpole/env.py:171-176,191-196): render errors are
-    swallowed by the env; the renderer itself raises."""
+def test_env_render_on_cpu_raises():
+    """No CPU fallback: the reference's env drops `pixels` when rendering
+    fails (cartpole/env.py:171-176,191-196); here the renderer's error
+    reaches the caller, for reset and for step."""
     env = pbr.envs.make("CartPole-v0", num_scenes=8, device="cpu")
+    with pytest.raises(RuntimeError, match="needs a GPU"):
+        env.reset()
+    env = pbr.envs.make("Steering-v0", num_scenes=4, device="cpu")
+    with pytest.raises(RuntimeError, match="needs a GPU"):
+        env.reset()
+    env = pbr.envs.make("CartPole-v0", num_scenes=8, device="cpu", render=False)
     td = env.reset()
-    assert "pixels" not in td
+    env.render = True
+    td["action"] = env.action_spec.rand()
+    with pytest.raises(RuntimeError, match="needs a GPU"):
+        env.step(td)
 
 
 def test_cartpole_dynamics_closed_form():
