@@ -361,6 +361,8 @@ def main() -> int:
     ap.add_argument("--loop-steps", type=int, default=1000)
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the back-to-back render launches (for counter profiles)")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "scene", "tiles", "chunked"],
+                    help="kernel family of the render call (PBRConfig.kernel)")
     ap.add_argument("--bwd-only", action="store_true",
                     help="only the fwd+bwd leg (for kernel profiles of the backward)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -379,10 +381,11 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     n = args.scenes
-    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda"))
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda", kernel=args.kernel))
     states = seeded_states(n, args.warmup + args.steps).cuda()
     if args.kernel_only:
         log("[bench] kernel timing only")
+        r.step(states[0])
         kus = kernel_time_us(r, launches=args.steps)
         print(json.dumps({"kernel_us": kus, "bytes_per_launch": algorithmic_bytes(r)}), flush=True)
         if args.copy_calibration:

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define PBR_ABI_VERSION 2
+#define PBR_ABI_VERSION 3
 #define PBR_MAX_NODES 16
 #define PBR_MAX_MESHES 16
 /* Largest tile edge in pixels: window positions in 1/256 pixel then stay
@@ -107,13 +107,17 @@ typedef struct pbr_frame_out {
 
 /* Launch shape.  Zero fields take library defaults. */
 enum pbr_kernel {
-  PBR_KERNEL_AUTO = 0,     /* scene kernel when every scene has <= 64 triangle
-                              slots, else the tile kernels                 */
+  PBR_KERNEL_AUTO = 0,     /* row-span kernel when every scene has <= 64
+                              triangle slots and tiles are <= 128 pixels
+                              wide, else the tile kernels                  */
   PBR_KERNEL_CHUNKED = 1,  /* any triangle count; region state in LDS      */
   PBR_KERNEL_SCENE = 2,    /* one wave per scene, <= 64 triangle slots     */
-  PBR_KERNEL_TILES = 3     /* any triangle count: setup once per triangle
+  PBR_KERNEL_TILES = 3,    /* any triangle count: setup once per triangle
                               into the workspace, then one workgroup per
                               (scene, square tile)                         */
+  PBR_KERNEL_ROWS = 4      /* <= 64 triangle slots, <= 128 columns: setup
+                              into LDS and exact per-row spans in one
+                              launch, no workspace                         */
 };
 
 typedef struct pbr_render_opts {
@@ -129,6 +133,10 @@ typedef struct pbr_render_opts {
                                  slots (sum over scenes of active instances
                                  x triangles); 0 = num_scenes x the largest
                                  scene's slots.  Sizes the workspace.       */
+  int32_t row_list_cap;       /* rows: row-list entries per band, 8..256,
+                                 0 = 256 (smaller values split the frame
+                                 into more bands)                           */
+  int32_t reserved;
 } pbr_render_opts;
 
 /* One pose component: value of row i = src ? src[i * stride] : value. */

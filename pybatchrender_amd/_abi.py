@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-PBR_ABI_VERSION = 2
+PBR_ABI_VERSION = 3
 PBR_MAX_NODES = 16
 PBR_MAX_MESHES = 16
 PBR_MAX_RESOLUTION = 4096
@@ -79,6 +79,7 @@ PBR_KERNEL_AUTO = 0
 PBR_KERNEL_CHUNKED = 1
 PBR_KERNEL_SCENE = 2
 PBR_KERNEL_TILES = 3
+PBR_KERNEL_ROWS = 4
 
 
 class pbr_render_opts(C.Structure):
@@ -92,6 +93,8 @@ class pbr_render_opts(C.Structure):
         ("lane_pixels", C.c_int32),
         ("tile_size", C.c_int32),
         ("slot_budget", C.c_int64),
+        ("row_list_cap", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -184,6 +187,7 @@ def expected_layout() -> list[int]:
         pbr_render_opts.kernel.offset,
         pbr_render_opts.bins_per_item.offset,
         pbr_render_opts.slot_budget.offset,
+        pbr_render_opts.row_list_cap.offset,
         C.sizeof(pbr_pose_component),
         C.sizeof(pbr_pose_desc),
         pbr_pose_desc.x.offset,

@@ -8,7 +8,8 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 SRCS = [PKG / "csrc" / "pbr_render.hip", PKG / "csrc" / "pbr_backward.hip",
-        PKG / "csrc" / "pbr_env.hip", PKG / "csrc" / "pbr_tiles.hip"]
+        PKG / "csrc" / "pbr_env.hip", PKG / "csrc" / "pbr_tiles.hip",
+        PKG / "csrc" / "pbr_rows.hip"]
 OUT = PKG / "libpbr_hip.so"
 
 HIPCC_FLAGS = [

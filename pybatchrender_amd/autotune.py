@@ -47,6 +47,9 @@ def max_slots(renderer) -> int:
 def candidates(renderer) -> list[dict]:
     """Launch shapes valid for this batch."""
     out = []
+    W = int(renderer.cfg.tile_resolution[0])
+    if max_slots(renderer) <= 64 and W <= 128:
+        out.append({"kernel": "rows"})
     if max_slots(renderer) <= 64:
         for bw in (16, 32, 64):
             for px in (4, 8):

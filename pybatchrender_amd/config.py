@@ -11,11 +11,14 @@ New fields (defaults keep the reference's behaviour):
     output_depth   also produce a [N, H, W] float32 window-depth tensor
     region_rows    chunked kernel: output rows per work item (0 = automatic)
     persistent     chunked kernel over a persistent (scene, region) work queue
-    kernel         'auto', 'scene' (one wave per scene, at most 64 triangle
-                   slots per scene), 'tiles' (any triangle count: one setup
-                   pass, then one workgroup per scene tile) or 'chunked' (any
-                   triangle count, setup repeated per row strip)
+    kernel         'auto', 'rows' (setup and exact per-row spans in one
+                   launch: at most 64 triangle slots per scene and tiles at
+                   most 128 pixels wide), 'scene' (one wave per scene, at most
+                   64 triangle slots per scene), 'tiles' (any triangle count:
+                   one setup pass, then one workgroup per scene tile) or
+                   'chunked' (any triangle count, setup repeated per row strip)
     tile_size      tile kernels: 16, 32 or 64 pixel square tiles (0 = 32)
+    row_list_cap   rows kernel: row-list entries per band, 8..256 (0 = 256)
     bin_width      scene kernel: 16, 32 or 64 pixel-wide bins (0 = automatic)
     bins_per_item  scene kernel: bins walked by one work item (0 = automatic)
     lane_pixels    scene kernel: 4 or 8 consecutive pixels per lane (0 = automatic)
@@ -87,6 +90,7 @@ class PBRConfig:
     bins_per_item: int = 0
     lane_pixels: int = 0
     tile_size: int = 0
+    row_list_cap: int = 0
 
     def process_resolution(self) -> None:
         """Atlas tiling rules of config.py:61-126 (cols = ceil(sqrt(N)))."""
@@ -176,10 +180,12 @@ class PBRConfig:
             raise ValueError("num_channels must be in [1, 4]")
         if self.region_rows < 0 or self.region_rows % 8:
             raise ValueError("region_rows must be 0 or a positive multiple of 8")
-        if self.kernel not in ("auto", "chunked", "scene", "tiles"):
+        if self.kernel not in ("auto", "chunked", "scene", "tiles", "rows"):
             raise ValueError(f"Invalid kernel: {self.kernel}")
         if self.tile_size not in (0, 16, 32, 64):
             raise ValueError("tile_size must be 0, 16, 32 or 64")
+        if self.row_list_cap != 0 and not 8 <= self.row_list_cap <= 256:
+            raise ValueError("row_list_cap must be 0 or within [8, 256]")
         if self.bin_width not in (0, 16, 32, 64):
             raise ValueError("bin_width must be 0, 16, 32 or 64")
         if self.lane_pixels not in (0, 4, 8):

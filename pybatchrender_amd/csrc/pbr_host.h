@@ -40,4 +40,10 @@ int tiles_workspace_bytes(const pbr_scene_desc* D, const pbr_render_opts* opts, 
 int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
                  void* workspace, size_t workspace_bytes, hipStream_t st);
 
+// The row-span kernel (pbr_rows.hip): whether it serves the descriptor, and
+// its launch (no workspace).
+bool rows_supported(const pbr_scene_desc* D);
+int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
+                hipStream_t st);
+
 }  // namespace pbr_host

@@ -970,14 +970,17 @@ SmallPlan small_plan(const pbr_scene_desc* D) {
   return p;
 }
 
-// The kernel family a call runs: AUTO takes the scene kernel for scenes of
-// at most 64 triangle slots (unless a persistent queue is asked for) and the
+// The kernel family a call runs: AUTO takes the row-span kernel for scenes of
+// at most 64 triangle slots in tiles at most 128 pixels wide (the scene
+// kernel for wider tiles) unless a persistent queue is asked for, and the
 // tile kernels otherwise.
 int choose_kernel(const pbr_scene_desc* D, const pbr_render_opts* opts) {
   const int kernel = opts ? opts->kernel : PBR_KERNEL_AUTO;
   if (kernel != PBR_KERNEL_AUTO) return kernel;
-  if (max_slots(D) <= (uint64_t)kSceneSlots && !(opts && opts->persistent))
-    return PBR_KERNEL_SCENE;
+  if (!(opts && opts->persistent)) {
+    if (rows_supported(D)) return PBR_KERNEL_ROWS;
+    if (max_slots(D) <= (uint64_t)kSceneSlots) return PBR_KERNEL_SCENE;
+  }
   return PBR_KERNEL_TILES;
 }
 
@@ -1122,6 +1125,7 @@ int pbr_abi_layout(int64_t* out, int32_t n) {
       (int64_t)offsetof(pbr_render_opts, kernel),
       (int64_t)offsetof(pbr_render_opts, bins_per_item),
       (int64_t)offsetof(pbr_render_opts, slot_budget),
+      (int64_t)offsetof(pbr_render_opts, row_list_cap),
       (int64_t)sizeof(pbr_pose_component),
       (int64_t)sizeof(pbr_pose_desc),
       (int64_t)offsetof(pbr_pose_desc, x),
@@ -1174,6 +1178,9 @@ int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts
       return PBR_OK;
     case PBR_KERNEL_TILES:
       return tiles_workspace_bytes(scene, opts, bytes);
+    case PBR_KERNEL_ROWS:
+      *bytes = 0;
+      return PBR_OK;
     default:
       *bytes = (opts && opts->persistent) ? 256 : 0;
       return PBR_OK;
@@ -1189,13 +1196,15 @@ int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out, const pbr_
   hipStream_t st = (hipStream_t)stream;
   const int kernel = opts ? opts->kernel : PBR_KERNEL_AUTO;
   if (kernel != PBR_KERNEL_AUTO && kernel != PBR_KERNEL_CHUNKED && kernel != PBR_KERNEL_SCENE &&
-      kernel != PBR_KERNEL_TILES)
+      kernel != PBR_KERNEL_TILES && kernel != PBR_KERNEL_ROWS)
     return fail(PBR_ERR_INVALID, "unknown kernel %d", kernel);
   switch (choose_kernel(scene, opts)) {
     case PBR_KERNEL_SCENE:
       return launch_small(scene, out, opts, workspace, workspace_bytes, st);
     case PBR_KERNEL_TILES:
       return launch_tiles(scene, out, opts, workspace, workspace_bytes, st);
+    case PBR_KERNEL_ROWS:
+      return launch_rows(scene, out, opts, st);
     default:
       return launch_chunked(scene, out, opts, workspace, workspace_bytes, st);
   }

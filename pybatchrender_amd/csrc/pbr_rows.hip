@@ -1,0 +1,925 @@
+// pbr_rows.hip - the row-span kernel: scenes of at most 64 triangle slots and
+// at most 128 pixel columns, setup and rasterisation in one launch, no
+// workspace.
+//
+// One workgroup of two waves takes two scenes.
+//   setup  one lane per (scene, slot) - both scenes share the first wave
+//          when they have at most 32 slots: transform (basic.vert), clip
+//          class, snap, cull, depth plane and flat colour -> a 64-byte record
+//          in LDS indexed by slot.  A triangle that straddles a clip plane
+//          keeps only the union of its sub-triangles' rows; its
+//          sub-triangles are re-derived where they are drawn.
+//   lists  (per wave = per scene, per band of rows) the (record, row) pairs
+//          of the band are dealt to lanes: a prefix sum over the records
+//          puts a marker at each record's first pair and a max-scan hands
+//          every lane its (record, row).  The lane computes the exact covered
+//          columns of the pair (pbr_span.h) and sets bit `record` of the
+//          row's count planes; a popcount scan over the rows then places
+//          every entry at
+//            row start + sum_b 2^b popcount(plane_b & lower records)
+//          so each row's list is in draw order (slot, sub-triangle).  The
+//          scans are DPP row shifts and broadcasts (no LDS round trips).
+//   rows   four rows at a time: 16 lanes per row, 4 (8) consecutive pixels
+//          per lane.  A lane walks its row's list (two entries in flight);
+//          inside an entry's span it takes the strictly smaller depth
+//          fmaf(zx, x + 0.5, tz), tz = fmaf(zy, y + 0.5, zc) - the same test
+//          in the same order as every other kernel of this library - with
+//          the entry's colour (uint8) or record (other outputs, smooth
+//          shading, visibility).  The winners are written with one dword
+//          (or wider) store per lane and channel straight into the NCHW
+//          outputs.
+//
+// Coverage, depth and colour are bit-identical to the CPU oracle.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pbr_abi.h"
+#include "pbr_device.h"
+#include "pbr_host.h"
+#include "pbr_span.h"
+
+using namespace pbr_host;
+
+namespace {
+
+constexpr int kRecs = 64;        // records (= slots) per scene
+constexpr int kBg = kRecs;       // table index of the background
+constexpr int kCap = 256;        // row-list entries per band
+constexpr int kBandRows = 64;    // rows per band at most
+constexpr int kRowsBlock = 128;  // two waves, two scenes
+
+constexpr uint32_t kRFlagSmooth = 8u;
+constexpr uint32_t kRFlagClipped = 16u;  // bits 5-7: sub-triangle count
+
+struct RowsFrame {
+  int W, H;
+  float guard;
+  int seg;  // setup lanes per scene (power of two >= slots, <= 64)
+  int cap;  // row-list entries per band (8..kCap)
+};
+
+// Unclipped: edge coefficients, depth plane and rows of a snapped triangle.
+// Clipped: A[0] = node, A[1] = slot within the node, rows = union of the
+// sub-triangles' rows.
+struct RowRec {
+  int32_t A[3], B[3];
+  double Cp[3];
+  float zx, zy, zc;
+  int16_t j0, j1;
+};
+static_assert(sizeof(RowRec) == 64, "RowRec is 64 bytes");
+
+template <int OM>
+struct ColOf;
+template <>
+struct ColOf<0> {  // uint8: packed RGBA8
+  using T = uint32_t;
+};
+template <>
+struct ColOf<1> {  // bf16: four bf16 values
+  using T = uint2;
+};
+template <>
+struct ColOf<2> {  // float32
+  using T = float4;
+};
+
+template <int OM>
+__device__ __forceinline__ typename ColOf<OM>::T pack_col(const float c[4]) {
+  if constexpr (OM == 0) {
+    return pack_rgba8(c);
+  } else if constexpr (OM == 1) {
+    return make_uint2(bf16_rne(c[0]) | (bf16_rne(c[1]) << 16),
+                      bf16_rne(c[2]) | (bf16_rne(c[3]) << 16));
+  } else {
+    return make_float4(c[0], c[1], c[2], c[3]);
+  }
+}
+
+template <bool SMOOTH, int OM>
+struct RowsLds {
+  RowRec rec[kRecs];
+  typename ColOf<OM>::T col[kRecs + 1];
+  uint8_t flags[kRecs];
+  unsigned long long plane[3][kBandRows];
+  uint32_t rowstart[kBandRows + 1];
+  union {
+    int32_t mark[kCap];  // pair -> record markers (consumed before entries are written)
+    uint4 ent[kCap];     // lo | hi << 8 | ref << 16, tz bits, zx bits, RGBA8 (uint8 output)
+  } u;
+  int nrec;
+  int any_clipped;
+  float sh[SMOOTH ? kRecs * kShadeFloats : 1];
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive scans over the 64 lanes: row shifts 1, 2, 4, 8 within each row of
+// 16 lanes, then row broadcasts of lanes 15 and 31 (identity where a lane
+// has no source).
+__device__ __forceinline__ int wave_scan_add(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+__device__ __forceinline__ int wave_scan_max(int v) {  // values >= -1
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false));
+  return v;
+}
+
+__device__ __forceinline__ int lane63(int v) { return __builtin_amdgcn_readlane(v, 63); }
+
+// Sub-triangle q (0-based) of a clipped triangle, re-derived exactly as at
+// setup.  False if it yields no fragment.
+__device__ __forceinline__ bool clipped_sub(const CVert cv[3], float guard, int W, int H, int q,
+                                            FixTri& T, CVert sub[3]) {
+  CVert poly[kMaxPoly];
+  poly[0] = cv[0];
+  poly[1] = cv[1];
+  poly[2] = cv[2];
+  const int n = clip_polygon(poly, guard);
+  if (q + 2 >= n) return false;
+  sub[0] = poly[0];
+  sub[1] = poly[q + 1];
+  sub[2] = poly[q + 2];
+  return setup_fixed(poly[0], poly[q + 1], poly[q + 2], W, H, 0, H - 1, T);
+}
+
+// Clip polygon of a triangle that straddles a clip plane: its vertex count,
+// and the union [j0, j1] of the GL rows its sub-triangles cover.
+__device__ __noinline__ int clipped_rows(const CVert* cv, RowsFrame F, int& j0, int& j1) {
+  CVert poly[kMaxPoly];
+  poly[0] = cv[0];
+  poly[1] = cv[1];
+  poly[2] = cv[2];
+  const int n = clip_polygon(poly, F.guard);
+  j0 = INT16_MAX;
+  j1 = -1;
+  for (int q = 1; q + 1 < n; ++q) {
+    FixTri S;
+    if (!setup_fixed(poly[0], poly[q], poly[q + 1], F.W, F.H, 0, F.H - 1, S)) continue;
+    j0 = min(j0, S.j0);
+    j1 = max(j1, S.j1);
+  }
+  return n;
+}
+
+// Non-empty spans at GL row j of the sub-triangles of a clipped triangle
+// (vertices cv), written to lo[]/hi[]/tz[]/zx[]/qs[] when `out` is set.
+// Returns their number.  Out of line: only triangles that straddle a clip
+// plane get here.
+__device__ __noinline__ int clipped_spans(const CVert* cv, RowsFrame F, int j, bool out, int* lo,
+                                          int* hi, float* tz, float* zx, int* qs) {
+  CVert poly[kMaxPoly];
+  poly[0] = cv[0];
+  poly[1] = cv[1];
+  poly[2] = cv[2];
+  const int n = clip_polygon(poly, F.guard);
+  int c = 0;
+  for (int q = 1; q + 1 < n; ++q) {
+    FixTri T;
+    if (!setup_fixed(poly[0], poly[q], poly[q + 1], F.W, F.H, 0, F.H - 1, T)) continue;
+    if (j < T.j0 || j > T.j1) continue;
+    pbr_span::Edges E;
+    pbr_span::make_edges(T.X, T.Y, T.bias, E);
+    int l, h;
+    pbr_span::row_span(E, j, F.W, l, h);
+    if (l > h) continue;
+    if (out) {
+      lo[c] = l;
+      hi[c] = h;
+      tz[c] = fmaf(T.zy, (float)j + 0.5f, T.zc);
+      zx[c] = T.zx;
+      qs[c] = q - 1;
+    }
+    ++c;
+  }
+  return c;
+}
+
+// Smooth colour of sub-triangle q of a clipped triangle (vertices cv,
+// instance colour rgba) at pixel centre (px, py), re-derived exactly as at
+// setup.  Out of line: only triangles that straddle a clip plane get here.
+__device__ __noinline__ void clipped_smooth_colour(const CVert* cv, const float* rgba, RowsFrame F,
+                                                   Light lt, int q, double px, double py,
+                                                   float* col) {
+  CVert sub[3];
+  FixTri T;
+  col[0] = col[1] = col[2] = col[3] = 0.0f;
+  if (!clipped_sub(cv, F.guard, F.W, F.H, q, T, sub)) return;
+  double A[3], B[3], C[3];
+  edges_from_fixed(T.X, T.Y, T.bias, A, B, C);
+  float sh[kShadeFloats];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sh[3 * k + c] = sub[k].n[c];
+    sh[9 + k] = T.invw[k];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sh[12 + c] = rgba[c];
+  shade_smooth(A, B, C, T.bias, sh, px, py, lt, col);
+}
+
+// Triangle `slot` of scene s: node, slot within the node and the scene's slot
+// count.  The node table walk is uniform (scalar loads of the descriptor);
+// only per-scene instance counts are read per lane.
+__device__ __forceinline__ int locate_slot(const pbr_scene_desc& D, int s, uint32_t slot,
+                                           uint32_t& local, uint32_t& total) {
+  int node = -1;
+  total = 0;
+  local = 0;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    const pbr_node& nd = D.nodes[j];
+    const int ips = nd.instances_per_scene;
+    int cnt = ips;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), ips);
+    const uint32_t n = (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
+    if (node < 0 && slot < total + n) {
+      node = j;
+      local = slot - total;
+    }
+    total += n;
+  }
+  return node;
+}
+
+// (a) of slot (node, local) of scene s: the node's fields are picked from a
+// uniform walk over the nodes, so the dependent loads are the instance row,
+// the scene's view-projection, the triangle's indices and then its vertices.
+__device__ __forceinline__ void fetch_slot(const pbr_scene_desc& D, int s, int node,
+                                           uint32_t local, CVert cv[3], float rgba[4],
+                                           bool& flat) {
+  const float* trans = nullptr;
+  const float* cols = nullptr;
+  int ips = 1, shared = 0, voff = 0, ioff = 0, ntri = 1, fl = 0;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    if (node == j) {
+      const pbr_node& nd = D.nodes[j];
+      const pbr_mesh& me = D.meshes[nd.mesh];
+      trans = nd.transforms;
+      cols = nd.colors;
+      ips = nd.instances_per_scene;
+      shared = nd.shared;
+      fl = nd.flat;
+      voff = me.vertex_offset;
+      ioff = me.index_offset;
+      ntri = me.num_triangles;
+    }
+  }
+  const uint32_t i = local / (uint32_t)ntri, t = local - i * (uint32_t)ntri;
+  const size_t row = shared ? (size_t)i : (size_t)s * (size_t)ips + (size_t)i;
+  const float4* M4 = reinterpret_cast<const float4*>(trans + row * 16);
+  const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2], m3 = M4[3];
+  const float M[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
+                       m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+  const float4 c4 = reinterpret_cast<const float4*>(cols)[row];
+  rgba[0] = c4.x;
+  rgba[1] = c4.y;
+  rgba[2] = c4.z;
+  rgba[3] = c4.w;
+  flat = fl != 0;
+  const float4* V4 = reinterpret_cast<const float4*>(D.view_proj + (size_t)s * 16);
+  const float4 q0 = V4[0], q1 = V4[1], q2 = V4[2], q3 = V4[3];
+  const float VP[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                        q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const int32_t* tri = D.indices + (size_t)(ioff + (int)t) * 3;
+  const int vi[3] = {tri[0], tri[1], tri[2]};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float* vv = D.vertices + (size_t)(voff + vi[k]) * 6;
+    float wp[4], nw[3];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      float a = M[rr * 4 + 0] * vv[0] + M[rr * 4 + 1] * vv[1];
+      a = a + M[rr * 4 + 2] * vv[2];
+      wp[rr] = a + M[rr * 4 + 3];
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      float a = VP[rr * 4 + 0] * wp[0] + VP[rr * 4 + 1] * wp[1];
+      a = a + VP[rr * 4 + 2] * wp[2];
+      cv[k].p[rr] = a + VP[rr * 4 + 3] * wp[3];
+    }
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      float a = M[rr * 4 + 0] * vv[3] + M[rr * 4 + 1] * vv[4];
+      nw[rr] = a + M[rr * 4 + 2] * vv[5];
+    }
+    normalize3(nw, cv[k].n);
+  }
+}
+
+__device__ __forceinline__ void edges_of(const FixTri& T, RowRec& R) {
+  pbr_span::Edges E;
+  pbr_span::make_edges(T.X, T.Y, T.bias, E);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    R.A[k] = E.A[k];
+    R.B[k] = E.B[k];
+    R.Cp[k] = E.Cp[k];
+  }
+  R.zx = T.zx;
+  R.zy = T.zy;
+  R.zc = T.zc;
+}
+
+// (a)+(b) of one slot of scene s into the scene's LDS tables.
+template <bool SMOOTH, int OM>
+__device__ __forceinline__ void setup_slot(const pbr_scene_desc& D, const RowsFrame& F,
+                                           const Light& lt, int s, uint32_t slot,
+                                           RowsLds<SMOOTH, OM>& L) {
+  uint32_t local, total;
+  const int node = locate_slot(D, s, slot, local, total);
+  if (slot == 0) L.nrec = (int)total;
+  if (node < 0) return;
+  CVert cv[3];
+  float rgba[4];
+  bool flat;
+  fetch_slot(D, s, node, local, cv, rgba, flat);
+  bool all_in, all_out;
+  clip_class(cv, F.guard, all_in, all_out);
+  RowRec R;
+  R.j0 = 1;
+  R.j1 = 0;
+  uint32_t fl = 0;
+  const bool smooth = SMOOTH && !flat;
+  FixTri T;
+  bool have_T = false;
+  if (!all_out) {
+    if (all_in) {
+      if (setup_fixed(cv[0], cv[1], cv[2], F.W, F.H, 0, F.H - 1, T)) {
+        edges_of(T, R);
+        R.j0 = (int16_t)T.j0;
+        R.j1 = (int16_t)T.j1;
+        fl = T.bias;
+        have_T = true;
+      }
+    } else {
+      int j0, j1;
+      const int n = clipped_rows(cv, F, j0, j1);
+      if (j0 <= j1) {
+        R.A[0] = node;
+        R.A[1] = (int32_t)local;
+        R.j0 = (int16_t)j0;
+        R.j1 = (int16_t)j1;
+        fl = kRFlagClipped | ((uint32_t)(n - 2) << 5);
+        atomicOr(&L.any_clipped, 1);
+      }
+    }
+  }
+  if (smooth) fl |= kRFlagSmooth;
+  L.rec[slot] = R;
+  L.flags[slot] = (uint8_t)fl;
+  if (!smooth) {
+    float flat_rgba[4];
+    shade(cv[0].n, rgba, lt, flat_rgba);
+    L.col[slot] = pack_col<OM>(flat_rgba);
+  } else if (SMOOTH && have_T) {
+    float* sh = &L.sh[slot * kShadeFloats];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sh[3 * k + c] = cv[k].n[c];
+      sh[9 + k] = T.invw[k];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sh[12 + c] = rgba[c];
+  }
+}
+
+// Colour of a smooth winner at pixel (x, j): per-pixel interpolation of the
+// (sub-)triangle's normals.
+template <bool SMOOTH, int OM>
+__device__ __forceinline__ void smooth_colour(const pbr_scene_desc& D, const RowsFrame& F,
+                                              const Light& lt, int s,
+                                              const RowsLds<SMOOTH, OM>& L, uint32_t ref, int x,
+                                              int j, float col[4]) {
+  const uint32_t slot = ref >> 3;
+  const uint32_t fl = L.flags[slot];
+  const RowRec& R = L.rec[slot];
+  const double px = (double)(256 * x + 128), py = (double)(256 * j + 128);
+  if (fl & kRFlagClipped) {
+    CVert cv[3];
+    float rgba[4];
+    bool flat;
+    fetch_slot(D, s, R.A[0], (uint32_t)R.A[1], cv, rgba, flat);
+    clipped_smooth_colour(cv, rgba, F, lt, (int)(ref & 7u), px, py, col);
+    return;
+  }
+  double A[3], B[3], C[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    A[k] = (double)R.A[k];
+    B[k] = (double)R.B[k];
+    C[k] = R.Cp[k] - 128.0 * A[k];
+  }
+  shade_smooth(A, B, C, fl & 7u, &L.sh[slot * kShadeFloats], px, py, lt, col);
+}
+
+// Row lists of GL rows [jb0, jb0 + R) for records [r0, r1) of the scene: on
+// return rowstart[0..R] and u.ent[] hold them.
+template <bool SMOOTH, int OM>
+__device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsFrame& F, int s,
+                                            RowsLds<SMOOTH, OM>& L, int lane, int jb0, int R,
+                                            int r0, int r1, int rj0, int rj1) {
+  const bool mine = lane >= r0 && lane < r1;
+  const int lo_row = max(rj0, jb0), hi_row = min(rj1, jb0 + R - 1);
+  const int cnt = mine ? max(0, hi_row - lo_row + 1) : 0;
+  const int incl = wave_scan_add(cnt);
+  const int st = incl - cnt;
+  const int P = min(lane63(incl), kCap);  // <= F.cap by the band choice
+  const bool clip = __builtin_amdgcn_readfirstlane(L.any_clipped) != 0;
+  for (int p = lane; p < P; p += 64) L.u.mark[p] = -1;
+  if (lane < R) {
+    L.plane[0][lane] = 0ull;
+    L.plane[1][lane] = 0ull;
+    L.plane[2][lane] = 0ull;
+  }
+  wave_sync();
+  // marker at the record's first pair: first pair (9 bits) | first row
+  // (14 bits) | record (6 bits); first pairs grow with the record, so the
+  // running maximum of the markers names the record of every pair
+  if (cnt > 0 && st < kCap) L.u.mark[st] = (st << 20) | (lo_row << 6) | lane;
+  wave_sync();
+
+  // pass A: each lane's pair -> span and count planes.  A lane keeps, per
+  // pair, record | band row << 8 (bit 16: has an entry) and lo | hi << 8.
+  constexpr int kIt = kCap / 64;
+  int prb[kIt], plh[kIt];
+  int carry = -1;
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    prb[it] = 0;
+    plh[it] = 0;
+    if (it * 64 >= P) continue;
+    const int p = it * 64 + lane;
+    int m = p < P ? L.u.mark[p] : -1;
+    m = max(wave_scan_max(m), carry);
+    carry = lane63(m);
+    if (p >= P) continue;
+    const int r = m & 63;
+    const int j = ((m >> 6) & 0x3fff) + (p - (m >> 20));
+    if (j < jb0 || j >= jb0 + R) continue;  // cannot happen for a well-formed marker
+    const RowRec& Rr = L.rec[r];
+    const uint32_t fl = L.flags[r];
+    int c = 0;
+    if (fl & kRFlagClipped) {
+      CVert cv[3];
+      float rgba[4];
+      bool flat;
+      fetch_slot(D, s, Rr.A[0], (uint32_t)Rr.A[1], cv, rgba, flat);
+      c = clipped_spans(cv, F, j, false, nullptr, nullptr, nullptr, nullptr, nullptr);
+    } else {
+      pbr_span::Edges E;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        E.A[k] = Rr.A[k];
+        E.B[k] = Rr.B[k];
+        E.Cp[k] = Rr.Cp[k];
+      }
+      int l, h;
+      pbr_span::row_span(E, j, F.W, l, h);
+      plh[it] = l | (h << 8);
+      c = l <= h ? 1 : 0;
+    }
+    if (c > 0) prb[it] = r | ((j - jb0) << 8) | (1 << 16);
+    const unsigned long long bit = 1ull << r;
+    if (c & 1) atomicOr(&L.plane[0][j - jb0], bit);
+    if (c & 2) atomicOr(&L.plane[1][j - jb0], bit);
+    if (c & 4) atomicOr(&L.plane[2][j - jb0], bit);
+  }
+  wave_sync();
+
+  // row starts: exclusive scan of the rows' entry counts
+  {
+    int c = 0;
+    if (lane < R) {
+      c = __popcll(L.plane[0][lane]);
+      if (clip) c += 2 * __popcll(L.plane[1][lane]) + 4 * __popcll(L.plane[2][lane]);
+    }
+    const int ic = wave_scan_add(c);
+    if (lane < R) L.rowstart[lane] = (uint32_t)(ic - c);
+    if (lane == R - 1) L.rowstart[R] = (uint32_t)ic;
+  }
+  wave_sync();
+
+  // pass B: entries at their draw-order positions
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    if (!(prb[it] >> 16)) continue;
+    const int r = prb[it] & 63, b = (prb[it] >> 8) & 0xff;
+    const unsigned long long below = (1ull << r) - 1ull;
+    uint32_t pos = L.rowstart[b] + (uint32_t)__popcll(L.plane[0][b] & below);
+    if (clip)
+      pos += 2u * (uint32_t)__popcll(L.plane[1][b] & below) +
+             4u * (uint32_t)__popcll(L.plane[2][b] & below);
+    uint32_t colw = 0u;
+    if constexpr (OM == 0) colw = L.col[r];
+    const RowRec& Rr = L.rec[r];
+    if (L.flags[r] & kRFlagClipped) {
+      int lo[7], hi[7], qs[7];
+      float tz[7], zx[7];
+      CVert cv[3];
+      float rgba[4];
+      bool flat;
+      fetch_slot(D, s, Rr.A[0], (uint32_t)Rr.A[1], cv, rgba, flat);
+      const int c = clipped_spans(cv, F, jb0 + b, true, lo, hi, tz, zx, qs);
+      for (int k = 0; k < c && pos + (uint32_t)k < (uint32_t)kCap; ++k)
+        L.u.ent[pos + k] = make_uint4(
+            (uint32_t)lo[k] | ((uint32_t)hi[k] << 8) |
+                ((((uint32_t)r << 3) | (uint32_t)qs[k]) << 16),
+            __float_as_uint(tz[k]), __float_as_uint(zx[k]), colw);
+    } else if (pos < (uint32_t)kCap) {
+      const float tz = fmaf(Rr.zy, (float)(jb0 + b) + 0.5f, Rr.zc);
+      L.u.ent[pos] = make_uint4((uint32_t)plh[it] | ((uint32_t)r << 19), __float_as_uint(tz),
+                                __float_as_uint(Rr.zx), colw);
+    }
+  }
+  wave_sync();
+}
+
+// Pixel state of one lane: PXL consecutive pixels of one row.
+template <int PXL>
+struct PixState {
+  float z[PXL];
+  uint32_t col[PXL];  // RGBA8 of the winner (uint8 output)
+  uint32_t ref[PXL];  // slot << 3 | sub-triangle of the winner
+};
+
+template <int PXL>
+__device__ __forceinline__ void init_state(PixState<PXL>& S, uint32_t bg) {
+#pragma unroll
+  for (int i = 0; i < PXL; ++i) {
+    S.z[i] = 1.0f;
+    S.col[i] = bg;
+    S.ref[i] = (uint32_t)kBg << 3;
+  }
+}
+
+// Depth test of one row-list entry on the lane's pixels (`ok` false: none).
+template <int PXL, bool OM0, bool REF>
+__device__ __forceinline__ void test_entry(const uint4& e, bool ok, int x0, const float* pxf,
+                                           PixState<PXL>& S) {
+  const int lo = ok ? (int)(e.x & 0xffu) : 255;
+  const int hi = (int)((e.x >> 8) & 0xffu);
+  const float tz = __uint_as_float(e.y), zx = __uint_as_float(e.z);
+  const uint32_t ref = e.x >> 16;
+#pragma unroll
+  for (int i = 0; i < PXL; ++i) {
+    const int x = x0 + i;
+    float z = fmaf(zx, pxf[i], tz);
+    z = fminf(fmaxf(z, 0.0f), 1.0f);
+    const bool upd = (x >= lo) & (x <= hi) & (z < S.z[i]);
+    S.z[i] = upd ? z : S.z[i];
+    if (OM0) S.col[i] = upd ? e.w : S.col[i];
+    if (REF) S.ref[i] = upd ? ref : S.ref[i];
+  }
+}
+
+// The lane's row-list entries [n0, n0 + cnt); `emax` (uniform) is the
+// longest list of the wave's rows.  Two entries are loaded per step (the
+// second is a no-op past the end of the list).
+template <int PXL, bool OM0, bool REF>
+__device__ __forceinline__ void walk_row(const uint4* ent, uint32_t n0, int cnt, int emax, int x0,
+                                         const float* pxf, PixState<PXL>& S) {
+  for (int k = 0; k < emax; k += 2) {
+    const bool ok0 = k < cnt, ok1 = k + 1 < cnt;
+    const uint4 e0 = ent[ok0 ? min(n0 + (uint32_t)k, (uint32_t)kCap - 1u) : 0u];
+    const uint4 e1 = ent[ok1 ? min(n0 + (uint32_t)k + 1u, (uint32_t)kCap - 1u) : 0u];
+    test_entry<PXL, OM0, REF>(e0, ok0, x0, pxf, S);
+    test_entry<PXL, OM0, REF>(e1, ok1, x0, pxf, S);
+  }
+}
+
+// byte c of each of four packed RGBA8 colours -> one dword
+__device__ __forceinline__ uint32_t gather_byte(uint32_t p0, uint32_t p1, uint32_t p2,
+                                                uint32_t p3, int c) {
+  const uint32_t sel = (uint32_t)c | ((uint32_t)(4 + c) << 8) | 0x0c0c0000u;
+  const uint32_t t01 = __builtin_amdgcn_perm(p1, p0, sel);
+  const uint32_t t23 = __builtin_amdgcn_perm(p3, p2, sel);
+  return __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+}
+
+// The output planes of one scene: every offset within them fits 32 bits.
+struct SceneOut {
+  void* color;     // [C][H][W] of the output dtype
+  float* depth;    // [H][W] or null
+  int32_t* vis;    // [H][W] or null
+  uint32_t plane;  // H * W
+  int C;
+};
+
+// Writes the lane's PXL pixels of GL row j (x0 .. x0 + PXL - 1, those < W).
+template <bool SMOOTH, int OM, int PXL>
+__device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const SceneOut& so,
+                                             const RowsFrame& F, const Light& lt, int s,
+                                             const RowsLds<SMOOTH, OM>& L, int x0, int j,
+                                             PixState<PXL>& S) {
+  if (x0 >= F.W) return;
+  const uint32_t pix0 = (uint32_t)(F.H - 1 - j) * (uint32_t)F.W + (uint32_t)x0;
+  typename ColOf<OM>::T cv[PXL];
+#pragma unroll
+  for (int i = 0; i < PXL; ++i) {
+    const uint32_t slot = S.ref[i] >> 3;
+    if constexpr (OM == 0) {
+      cv[i] = S.col[i];
+    } else {
+      cv[i] = L.col[slot];
+    }
+    if (SMOOTH && slot != (uint32_t)kBg && (L.flags[slot] & kRFlagSmooth)) {
+      float c4[4];
+      smooth_colour<SMOOTH, OM>(D, F, lt, s, L, S.ref[i], x0 + i, j, c4);
+      cv[i] = pack_col<OM>(c4);
+    }
+  }
+  if ((F.W % PXL) == 0) {
+    // whole, aligned lane pixels: one dword (or wider) store per channel
+    if constexpr (OM == 0) {
+      uint8_t* dst = reinterpret_cast<uint8_t*>(so.color);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= so.C) break;
+#pragma unroll
+        for (int h = 0; h < PXL; h += 4)
+          *reinterpret_cast<uint32_t*>(dst + (size_t)c * so.plane + pix0 + h) =
+              gather_byte(cv[h], cv[h + 1], cv[h + 2], cv[h + 3], c);
+      }
+    } else if constexpr (OM == 1) {
+      uint16_t* dst = reinterpret_cast<uint16_t*>(so.color);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= so.C) break;
+        const uint32_t sel = (c & 1) ? 0x07060302u : 0x05040100u;
+#pragma unroll
+        for (int h = 0; h < PXL; h += 2) {
+          const uint32_t a = (c < 2) ? cv[h].x : cv[h].y;
+          const uint32_t b = (c < 2) ? cv[h + 1].x : cv[h + 1].y;
+          *reinterpret_cast<uint32_t*>(dst + (size_t)c * so.plane + pix0 + h) =
+              __builtin_amdgcn_perm(b, a, sel);
+        }
+      }
+    } else {
+      float* dst = reinterpret_cast<float*>(so.color);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= so.C) break;
+#pragma unroll
+        for (int h = 0; h < PXL; h += 4) {
+          float v[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            v[k] = c == 0 ? cv[h + k].x : (c == 1 ? cv[h + k].y : (c == 2 ? cv[h + k].z : cv[h + k].w));
+          *reinterpret_cast<float4*>(dst + (size_t)c * so.plane + pix0 + h) =
+              make_float4(v[0], v[1], v[2], v[3]);
+        }
+      }
+    }
+    if (so.depth) {
+#pragma unroll
+      for (int h = 0; h < PXL; h += 4)
+        *reinterpret_cast<float4*>(so.depth + pix0 + h) =
+            make_float4(S.z[h] + 0.0f, S.z[h + 1] + 0.0f, S.z[h + 2] + 0.0f, S.z[h + 3] + 0.0f);
+    }
+    if (so.vis) {
+      int32_t v[PXL];
+#pragma unroll
+      for (int i = 0; i < PXL; ++i) v[i] = S.z[i] < 1.0f ? (int32_t)(S.ref[i] >> 3) : -1;
+#pragma unroll
+      for (int h = 0; h < PXL; h += 4)
+        *reinterpret_cast<int4*>(so.vis + pix0 + h) = make_int4(v[h], v[h + 1], v[h + 2], v[h + 3]);
+    }
+    return;
+  }
+  // ragged tile width: pixel by pixel
+#pragma unroll
+  for (int i = 0; i < PXL; ++i) {
+    if (x0 + i >= F.W) break;
+    const uint32_t pix = pix0 + (uint32_t)i;
+    if constexpr (OM == 0) {
+      uint8_t* dst = reinterpret_cast<uint8_t*>(so.color);
+      for (int c = 0; c < so.C; ++c) dst[(size_t)c * so.plane + pix] = (uint8_t)(cv[i] >> (8 * c));
+    } else if constexpr (OM == 1) {
+      uint16_t* dst = reinterpret_cast<uint16_t*>(so.color);
+      const uint32_t h4[4] = {cv[i].x & 0xffffu, cv[i].x >> 16, cv[i].y & 0xffffu, cv[i].y >> 16};
+      for (int c = 0; c < so.C; ++c) dst[(size_t)c * so.plane + pix] = (uint16_t)h4[c];
+    } else {
+      float* dst = reinterpret_cast<float*>(so.color);
+      const float f4[4] = {cv[i].x, cv[i].y, cv[i].z, cv[i].w};
+      for (int c = 0; c < so.C; ++c) dst[(size_t)c * so.plane + pix] = f4[c];
+    }
+    if (so.depth) so.depth[pix] = S.z[i] + 0.0f;
+    if (so.vis) so.vis[pix] = S.z[i] < 1.0f ? (int32_t)(S.ref[i] >> 3) : -1;
+  }
+}
+
+template <bool SMOOTH, int OM, int PXL, bool REF>
+__device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_frame_out& O,
+                                             const RowsFrame& F, const Light& lt, int s,
+                                             RowsLds<SMOOTH, OM>& L, int lane) {
+  constexpr bool OM0 = OM == 0;
+  const int nrec = __builtin_amdgcn_readfirstlane(L.nrec);
+  int rj0 = 1, rj1 = 0, rw = 1;
+  if (lane < nrec) {
+    rj0 = L.rec[lane].j0;
+    rj1 = L.rec[lane].j1;
+    const uint32_t fl = L.flags[lane];
+    rw = (fl & kRFlagClipped) ? (int)(fl >> 5) : 1;
+  }
+  const int H = F.H;
+  SceneOut so;
+  so.plane = (uint32_t)H * (uint32_t)F.W;
+  so.C = O.channels;
+  const size_t esize = OM == 0 ? 1 : (OM == 1 ? 2 : 4);
+  so.color = reinterpret_cast<char*>(O.color) + (size_t)s * (size_t)so.C * so.plane * esize;
+  so.depth = O.depth ? O.depth + (size_t)s * so.plane : nullptr;
+  so.vis = O.vis ? O.vis + (size_t)s * so.plane : nullptr;
+  // lane -> (row q of a group of four, pixels x0 .. x0 + PXL - 1)
+  const int q = lane >> 4;
+  for (int jb0 = 0; jb0 < H;) {
+    // band: at most kBandRows rows whose entries fit the list
+    int R = min(kBandRows, H - jb0);
+    int E;
+    for (;;) {
+      const int lo_row = max(rj0, jb0), hi_row = min(rj1, jb0 + R - 1);
+      const int cnt = lane < nrec ? max(0, hi_row - lo_row + 1) : 0;
+      E = lane63(wave_scan_add(cnt * rw));
+      if (E <= F.cap || R == 1) break;
+      R = (R + 1) >> 1;
+    }
+    if (E <= F.cap) {
+      build_lists<SMOOTH, OM>(D, F, s, L, lane, jb0, R, 0, nrec, rj0, rj1);
+      // per-lane constants of the row walk, made after the lists so they are
+      // not held through them
+      const int x0 = (lane & 15) * PXL;
+      float pxf[PXL];
+#pragma unroll
+      for (int i = 0; i < PXL; ++i) pxf[i] = (float)(x0 + i) + 0.5f;
+      uint32_t bg = 0u;
+      if constexpr (OM0) bg = L.col[kBg];
+      for (int g = 0; g < R; g += 4) {
+        const int b = g + q;
+        const bool row_ok = b < R;
+        uint32_t n0 = 0, n1 = 0;
+        if (row_ok) {
+          n0 = L.rowstart[b];
+          n1 = L.rowstart[b + 1];
+        }
+        const int cnt = (int)(n1 - n0);
+        const int emax =
+            max(max(__builtin_amdgcn_readlane(cnt, 0), __builtin_amdgcn_readlane(cnt, 16)),
+                max(__builtin_amdgcn_readlane(cnt, 32), __builtin_amdgcn_readlane(cnt, 48)));
+        PixState<PXL> S;
+        init_state<PXL>(S, bg);
+        walk_row<PXL, OM0, REF>(L.u.ent, n0, cnt, emax, x0, pxf, S);
+        if (row_ok) resolve_lane<SMOOTH, OM, PXL>(D, so, F, lt, s, L, x0, jb0 + b, S);
+      }
+      wave_sync();
+    } else {
+      // one row whose entries exceed the list: records in chunks, the
+      // pixel state of the row (lanes of group row 0) carried across them
+      const int j = jb0;
+      const int x0 = (lane & 15) * PXL;
+      float pxf[PXL];
+#pragma unroll
+      for (int i = 0; i < PXL; ++i) pxf[i] = (float)(x0 + i) + 0.5f;
+      uint32_t bg = 0u;
+      if constexpr (OM0) bg = L.col[kBg];
+      PixState<PXL> S;
+      init_state<PXL>(S, bg);
+      const int has = (lane < nrec && rj0 <= j && j <= rj1) ? rw : 0;
+      const int incl = wave_scan_add(has);
+      const int chunk = (incl - has) / (F.cap - 7);
+      const int nchunk = lane63(chunk) + 1;
+      for (int c = 0; c < nchunk; ++c) {
+        const uint64_t m = __ballot(chunk == c);
+        if (!m) continue;
+        const int r0 = __ffsll((unsigned long long)m) - 1;
+        const int r1 = 64 - __clzll((long long)m);
+        build_lists<SMOOTH, OM>(D, F, s, L, lane, j, 1, r0, r1, rj0, rj1);
+        const uint32_t n0 = L.rowstart[0];
+        const int cnt = q == 0 ? (int)(L.rowstart[1] - n0) : 0;
+        const int emax = __builtin_amdgcn_readlane(cnt, 0);
+        walk_row<PXL, OM0, REF>(L.u.ent, n0, cnt, emax, x0, pxf, S);
+        wave_sync();
+      }
+      if (q == 0) resolve_lane<SMOOTH, OM, PXL>(D, so, F, lt, s, L, x0, j, S);
+    }
+    jb0 += R;
+  }
+}
+
+template <bool SMOOTH, int OM, int PXL, bool REF>
+__global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, pbr_frame_out O,
+                                                             RowsFrame F, Light lt, float4 bg) {
+  __shared__ RowsLds<SMOOTH, OM> lds[2];
+  const int tid = (int)threadIdx.x;
+  // the wave index is uniform within the wave: keep it (and what derives
+  // from it - scene, LDS region, output bases) in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int s0 = (int)blockIdx.x * 2;
+  if (lane == 0) lds[wave].any_clipped = 0;
+  if (lane == 1) {
+    const float bgc[4] = {bg.x, bg.y, bg.z, bg.w};
+    lds[wave].col[kBg] = pack_col<OM>(bgc);
+  }
+  __syncthreads();
+  {
+    // setup lanes: tid -> (scene s0 + tid / seg, slot tid % seg)
+    const int sl = tid / F.seg;
+    const uint32_t slot = (uint32_t)(tid % F.seg);
+    const int s = s0 + sl;
+    if (sl < 2) {
+      if (s < D.num_scenes)
+        setup_slot<SMOOTH, OM>(D, F, lt, s, slot, lds[sl]);
+      else if (slot == 0)
+        lds[sl].nrec = 0;
+    }
+  }
+  __syncthreads();
+  const int s = s0 + wave;
+  if (s >= D.num_scenes) return;
+  raster_scene<SMOOTH, OM, PXL, REF>(D, O, F, lt, s, lds[wave], lane);
+}
+
+template <bool SMOOTH, int OM, bool REF>
+hipError_t launch_rows_t(const pbr_scene_desc* D, const pbr_frame_out* O, const RowsFrame& F,
+                         const Light& lt, float4 bg, hipStream_t st) {
+  const int grid = (D->num_scenes + 1) / 2;
+  if (D->width <= 64)
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 4, REF>), dim3(grid), dim3(kRowsBlock), 0, st, *D,
+                       *O, F, lt, bg);
+  else
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 8, REF>), dim3(grid), dim3(kRowsBlock), 0, st, *D,
+                       *O, F, lt, bg);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace pbr_host {
+
+bool rows_supported(const pbr_scene_desc* D) {
+  return max_slots(D) <= (uint64_t)kRecs && D->width <= 128 && D->height <= 16384;
+}
+
+int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
+                hipStream_t st) {
+  if (max_slots(D) > (uint64_t)kRecs)
+    return fail(PBR_ERR_UNSUPPORTED, "row-span kernel needs at most %d triangle slots per scene",
+                kRecs);
+  if (D->width > 128 || D->height > 16384)
+    return fail(PBR_ERR_UNSUPPORTED, "row-span kernel needs tiles of at most 128 x 16384 pixels");
+  RowsFrame F;
+  F.W = D->width;
+  F.H = D->height;
+  F.guard = 65536.0f / (float)(D->width > D->height ? D->width : D->height);
+  const int slots = (int)max_slots(D);
+  F.seg = 1;
+  while (F.seg < slots) F.seg <<= 1;
+  const int cap = opts ? opts->row_list_cap : 0;
+  if (cap != 0 && (cap < 8 || cap > kCap))
+    return fail(PBR_ERR_INVALID, "row_list_cap %d outside [8, %d]", cap, kCap);
+  F.cap = cap ? cap : kCap;
+  const Light lt = host_light(D);
+  const float4 bg = make_float4(D->background[0], D->background[1], D->background[2],
+                                D->background[3]);
+  const bool smooth = any_smooth(D);
+  hipError_t e;
+  switch (O->dtype) {
+    case PBR_DTYPE_U8:
+      if (smooth)
+        e = launch_rows_t<true, 0, true>(D, O, F, lt, bg, st);
+      else if (O->vis)
+        e = launch_rows_t<false, 0, true>(D, O, F, lt, bg, st);
+      else
+        e = launch_rows_t<false, 0, false>(D, O, F, lt, bg, st);
+      break;
+    case PBR_DTYPE_BF16:
+      e = smooth ? launch_rows_t<true, 1, true>(D, O, F, lt, bg, st)
+                 : launch_rows_t<false, 1, true>(D, O, F, lt, bg, st);
+      break;
+    default:
+      e = smooth ? launch_rows_t<true, 2, true>(D, O, F, lt, bg, st)
+                 : launch_rows_t<false, 2, true>(D, O, F, lt, bg, st);
+      break;
+  }
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "row-span launch failed: %s", hipGetErrorString(e));
+  return PBR_OK;
+}
+
+}  // namespace pbr_host

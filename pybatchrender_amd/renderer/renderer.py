@@ -267,12 +267,14 @@ class PBRRenderer:
         o.persistent = 1 if self.cfg.persistent else 0
         o.workgroups = int(getattr(self, "workgroups", 0))  # persistent launch size (autotune)
         o.kernel = {"auto": _abi.PBR_KERNEL_AUTO, "chunked": _abi.PBR_KERNEL_CHUNKED,
-                    "scene": _abi.PBR_KERNEL_SCENE, "tiles": _abi.PBR_KERNEL_TILES}[self.cfg.kernel]
+                    "scene": _abi.PBR_KERNEL_SCENE, "tiles": _abi.PBR_KERNEL_TILES,
+                    "rows": _abi.PBR_KERNEL_ROWS}[self.cfg.kernel]
         o.bin_width = int(self.cfg.bin_width)
         o.bins_per_item = int(self.cfg.bins_per_item)
         o.lane_pixels = int(self.cfg.lane_pixels)
         o.tile_size = int(self.cfg.tile_size)
         o.slot_budget = self.slot_budget()
+        o.row_list_cap = int(self.cfg.row_list_cap)
         return o
 
     def slot_budget(self) -> int:

@@ -3,6 +3,9 @@ import ctypes
 import re
 from pathlib import Path
 
+import pytest
+
+import pybatchrender_amd as pbr
 from pybatchrender_amd import _abi, _native
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -96,3 +99,25 @@ def test_workspace_size():
     assert _native.workspace_bytes(d, o) == want
     o.slot_budget = 50
     assert _native.workspace_bytes(d, o) < want
+
+
+def test_rows_kernel_needs_no_workspace():
+    d = _abi.pbr_scene_desc()
+    o = _abi.pbr_render_opts()
+    d.num_scenes, d.width, d.height = 4096, 64, 64
+    d.num_nodes, d.num_meshes = 1, 1
+    d.meshes[0].num_triangles = 12
+    d.nodes[0].instances_per_scene = 2
+    assert _native.workspace_bytes(d, o) == 0  # AUTO picks the row-span kernel
+    o.kernel = _abi.PBR_KERNEL_ROWS
+    assert _native.workspace_bytes(d, o) == 0
+    o.kernel = _abi.PBR_KERNEL_SCENE
+    assert _native.workspace_bytes(d, o) > 0
+
+
+def test_rows_list_cap_is_validated():
+    with pytest.raises(ValueError):
+        pbr.PBRConfig(num_scenes=1, device="cpu", row_list_cap=4)
+    with pytest.raises(ValueError):
+        pbr.PBRConfig(num_scenes=1, device="cpu", row_list_cap=512)
+    assert pbr.PBRConfig(num_scenes=1, device="cpu", row_list_cap=8).row_list_cap == 8

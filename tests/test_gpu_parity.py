@@ -176,7 +176,7 @@ def test_smooth_geometry_through_the_near_plane(cuda, kernel):
     check_parity(r, f"smooth near-plane clipping ({kernel})")
 
 
-@pytest.mark.parametrize("kernel", ["tiles", "chunked"])
+@pytest.mark.parametrize("kernel", ["tiles", "chunked", "rows"])
 def test_near_plane_and_guard_band_clipping(cuda, kernel):
     """Geometry that crosses the near plane and extends far past the view."""
     n = 16
@@ -299,11 +299,113 @@ def test_scene_kernel_heavy_clipping_multi_pass(cuda):
     check_parity(r, "scene kernel clipping")
 
 
-@pytest.mark.parametrize("kernel", ["scene", "chunked", "tiles"])
+@pytest.mark.parametrize("kernel", ["scene", "chunked", "tiles", "rows"])
 def test_cartpole_both_kernels(cuda, kernel):
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=512, device="cuda", kernel=kernel))
     r._step(ground_states(512, 5).cuda())
     check_parity(r, f"cartpole ground {kernel}")
+
+
+# --- the row-span kernel (<= 64 slots, <= 128 columns) against the oracle -----------
+@pytest.mark.parametrize("res", [(64, 64), (17, 13), (1, 1), (84, 84), (128, 70), (64, 200)])
+def test_rows_kernel_resolutions(cuda, res):
+    r = _small_scene(12, res, 31, kernel="rows")
+    check_parity(r, f"rows kernel {res}")
+
+
+@pytest.mark.parametrize("cap", [8, 16, 64])
+def test_rows_kernel_short_lists(cuda, cap):
+    """A short row list forces many bands and, at 8 entries, single rows whose
+    entries are built in record chunks with the pixel state carried over."""
+    r = _small_scene(16, (64, 64), 32, kernel="rows", row_list_cap=cap)
+    check_parity(r, f"rows kernel list cap {cap}")
+
+
+@pytest.mark.parametrize("dtype,channels", [("float32", 4), ("bfloat16", 3), ("uint8", 2),
+                                            ("uint8", 1)])
+def test_rows_kernel_smooth_dtypes(cuda, dtype, channels):
+    from pybatchrender_amd.renderer import mesh
+
+    low = mesh.uv_sphere(segments=6, rings=4)  # 36 smooth triangles
+    r = _small_scene(24, (40, 40), 33, model=low, instances=1, kernel="rows",
+                     output_dtype=dtype, num_channels=channels)
+    check_parity(r, f"rows kernel smooth {dtype} C={channels}")
+
+
+def test_rows_kernel_overlapping_full_frame(cuda):
+    """Thirty frame-filling quads, some at exactly equal depth: every row
+    holds dozens of entries (many bands) and ties go to the earlier slot."""
+    n = 6
+    r = PBRRenderer(PBRConfig(num_scenes=n, tile_resolution=(64, 64), device="cuda",
+                              kernel="rows"))
+    node = r.add_node("models/quad", instances_per_scene=30, model_scale=(8.0, 8.0, 1.0),
+                      model_pivot_relative_point=(0.5, 0.5, 0.0))
+    depth = torch.tensor([float(k // 2) * 0.2 for k in range(30)])  # pairs share a depth
+    M = torch.eye(4).repeat(n * 30, 1, 1)
+    M[:, :3, :3] = torch.tensor([[1.0, 0.0, 0.0], [0.0, 0.0, -1.0], [0.0, 1.0, 0.0]])  # face -y
+    M[:, 1, 3] = depth.repeat(n)
+    node.set_transforms(M)
+    gen = torch.Generator("cpu").manual_seed(34)
+    node.set_colors(torch.rand(n * 30, 4, generator=gen))
+    r.add_camera()
+    eye = torch.zeros(n, 3)
+    eye[:, 1] = -3.0 - 0.5 * torch.arange(n, dtype=torch.float32)
+    eye[:, 0] = 0.05 * torch.arange(n, dtype=torch.float32)
+    eye[:, 2] = 0.2
+    r._pbr_cam.set_positions_and_lookat(eye, torch.tensor([0.0, 10.0, 0.0]))
+    r.add_light()
+    r.setup_environment()
+    out = check_parity(r, "rows kernel overlapping quads")
+    assert (out["got"]["vis"] >= 0).float().mean() > 0.9
+
+
+@pytest.mark.parametrize("cap", [0, 8])
+def test_rows_kernel_heavy_clipping(cuda, cap):
+    """Large boxes around the camera: most triangles cross the near plane and
+    are re-derived into sub-triangles where they are drawn."""
+    n = 8
+    gen = torch.Generator("cpu").manual_seed(35)
+    r = PBRRenderer(PBRConfig(num_scenes=n, tile_resolution=(48, 48), device="cuda",
+                              kernel="rows", row_list_cap=cap))
+    node = r.add_node("models/box", instances_per_scene=5, model_pivot_relative_point=(0.5, 0.5, 0.5))
+    node.set_transforms(workloads.random_transforms(n * 5, gen, pos_range=1.5,
+                                                    scale_range=(3.0, 8.0)))
+    node.set_colors(torch.rand(n * 5, 4, generator=gen))
+    r.add_camera(z_near=0.3)
+    r._pbr_cam.set_positions_and_lookat(torch.zeros(3), torch.tensor([0.3, 1.0, 0.2]))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, f"rows kernel clipping cap {cap}")
+
+
+def test_rows_kernel_smooth_through_the_near_plane(cuda):
+    """Low-poly spheres around the camera: clipped smooth sub-triangles are
+    shaded from their re-derived vertices."""
+    from pybatchrender_amd.renderer import mesh
+
+    n = 10
+    low = mesh.uv_sphere(segments=6, rings=4)
+    c = PBRConfig(num_scenes=n, tile_resolution=(48, 48), device="cuda", kernel="rows",
+                  output_dtype="float32", output_depth=True)
+    r = PBRRenderer(c)
+    sph = r.add_node(low, instances_per_scene=1)
+    gen = torch.Generator("cpu").manual_seed(36)
+    M = workloads.random_transforms(n, gen, pos_range=0.5, scale_range=(2.0, 4.0))
+    sph.set_transforms(M)
+    sph.set_colors(torch.rand(n, 4, generator=gen))
+    r.add_camera(fov_y_deg=80.0, z_near=0.8, z_far=40.0)
+    eye = torch.zeros(n, 3)
+    eye[:, 1] = -1.0 + 0.1 * torch.arange(n)
+    r._pbr_cam.set_positions_and_lookat(eye, torch.tensor([0.0, 5.0, 0.0]))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, "rows kernel smooth near-plane clipping")
+
+
+def test_auto_beyond_rows_limits(cuda):
+    """Tiles wider than 128 pixels: AUTO takes the scene kernel instead."""
+    r = _small_scene(6, (130, 40), 37)
+    check_parity(r, "auto at 130 columns")
 
 
 # --- pose kernel and graph-captured steps -------------------------------------------
