@@ -2,17 +2,17 @@
 // at most 128 pixel columns, setup and rasterisation in one launch, no
 // workspace.
 //
-// One workgroup of two waves takes two scenes.
+// One workgroup of four waves takes two scenes, two waves per scene.
 //   setup  one lane per (scene, slot) - both scenes share the first wave
 //          when they have at most 32 slots: transform (basic.vert), clip
 //          class, snap, cull, depth plane and flat colour -> a 64-byte record
 //          in LDS indexed by slot.  A triangle that straddles a clip plane
 //          keeps only the union of its sub-triangles' rows; its
 //          sub-triangles are re-derived where they are drawn.
-//   lists  (per wave = per scene, per band of rows) the (record, row) pairs
-//          of the band are dealt to lanes: a prefix sum over the records
-//          puts a marker at each record's first pair and a max-scan hands
-//          every lane its (record, row).  The lane computes the exact covered
+//   lists  (per wave = per half of a scene's rows, per band of rows) the
+//          (record, row) pairs of the band are dealt to lanes: a prefix sum
+//          over the records puts a marker at each record's first pair and a
+//          max-scan hands every lane its (record, row).  The lane computes the exact covered
 //          columns of the pair (pbr_span.h) and sets bit `record` of the
 //          row's count planes; a popcount scan over the rows then places
 //          every entry at
@@ -46,7 +46,10 @@ constexpr int kRecs = 64;        // records (= slots) per scene
 constexpr int kBg = kRecs;       // table index of the background
 constexpr int kCap = 256;        // row-list entries per band
 constexpr int kBandRows = 64;    // rows per band at most
-constexpr int kRowsBlock = 128;  // two waves, two scenes
+// Waves sharing a scene's rows (measured on MI355X for 64 x 64 CartPole
+// frames: 2 -> 36 us, 4 -> 51 us, 1 -> 42 us per 4096-scene call).
+constexpr int kWavesPerScene = 2;
+constexpr int kRowsBlock = 128 * kWavesPerScene;  // two scenes per workgroup
 
 constexpr uint32_t kRFlagSmooth = 8u;
 constexpr uint32_t kRFlagClipped = 16u;  // bits 5-7: sub-triangle count
@@ -96,20 +99,25 @@ __device__ __forceinline__ typename ColOf<OM>::T pack_col(const float c[4]) {
   }
 }
 
+// A scene's triangle records and colour table (shared by the scene's waves).
 template <bool SMOOTH, int OM>
 struct RowsLds {
   RowRec rec[kRecs];
   typename ColOf<OM>::T col[kRecs + 1];
   uint8_t flags[kRecs];
+  int nrec;
+  int any_clipped;
+  float sh[SMOOTH ? kRecs * kShadeFloats : 1];
+};
+
+// One wave's row lists.
+struct ListLds {
   unsigned long long plane[3][kBandRows];
   uint32_t rowstart[kBandRows + 1];
   union {
     int32_t mark[kCap];  // pair -> record markers (consumed before entries are written)
     uint4 ent[kCap];     // lo | hi << 8 | ref << 16, tz bits, zx bits, RGBA8 (uint8 output)
   } u;
-  int nrec;
-  int any_clipped;
-  float sh[SMOOTH ? kRecs * kShadeFloats : 1];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -435,8 +443,8 @@ __device__ __forceinline__ void smooth_colour(const pbr_scene_desc& D, const Row
 // return rowstart[0..R] and u.ent[] hold them.
 template <bool SMOOTH, int OM>
 __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsFrame& F, int s,
-                                            RowsLds<SMOOTH, OM>& L, int lane, int jb0, int R,
-                                            int r0, int r1, int rj0, int rj1) {
+                                            const RowsLds<SMOOTH, OM>& L, ListLds& Q, int lane,
+                                            int jb0, int R, int r0, int r1, int rj0, int rj1) {
   const bool mine = lane >= r0 && lane < r1;
   const int lo_row = max(rj0, jb0), hi_row = min(rj1, jb0 + R - 1);
   const int cnt = mine ? max(0, hi_row - lo_row + 1) : 0;
@@ -444,17 +452,17 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
   const int st = incl - cnt;
   const int P = min(lane63(incl), kCap);  // <= F.cap by the band choice
   const bool clip = __builtin_amdgcn_readfirstlane(L.any_clipped) != 0;
-  for (int p = lane; p < P; p += 64) L.u.mark[p] = -1;
+  for (int p = lane; p < P; p += 64) Q.u.mark[p] = -1;
   if (lane < R) {
-    L.plane[0][lane] = 0ull;
-    L.plane[1][lane] = 0ull;
-    L.plane[2][lane] = 0ull;
+    Q.plane[0][lane] = 0ull;
+    Q.plane[1][lane] = 0ull;
+    Q.plane[2][lane] = 0ull;
   }
   wave_sync();
   // marker at the record's first pair: first pair (9 bits) | first row
   // (14 bits) | record (6 bits); first pairs grow with the record, so the
   // running maximum of the markers names the record of every pair
-  if (cnt > 0 && st < kCap) L.u.mark[st] = (st << 20) | (lo_row << 6) | lane;
+  if (cnt > 0 && st < kCap) Q.u.mark[st] = (st << 20) | (lo_row << 6) | lane;
   wave_sync();
 
   // pass A: each lane's pair -> span and count planes.  A lane keeps, per
@@ -468,7 +476,7 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
     plh[it] = 0;
     if (it * 64 >= P) continue;
     const int p = it * 64 + lane;
-    int m = p < P ? L.u.mark[p] : -1;
+    int m = p < P ? Q.u.mark[p] : -1;
     m = max(wave_scan_max(m), carry);
     carry = lane63(m);
     if (p >= P) continue;
@@ -499,9 +507,9 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
     }
     if (c > 0) prb[it] = r | ((j - jb0) << 8) | (1 << 16);
     const unsigned long long bit = 1ull << r;
-    if (c & 1) atomicOr(&L.plane[0][j - jb0], bit);
-    if (c & 2) atomicOr(&L.plane[1][j - jb0], bit);
-    if (c & 4) atomicOr(&L.plane[2][j - jb0], bit);
+    if (c & 1) atomicOr(&Q.plane[0][j - jb0], bit);
+    if (c & 2) atomicOr(&Q.plane[1][j - jb0], bit);
+    if (c & 4) atomicOr(&Q.plane[2][j - jb0], bit);
   }
   wave_sync();
 
@@ -509,12 +517,12 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
   {
     int c = 0;
     if (lane < R) {
-      c = __popcll(L.plane[0][lane]);
-      if (clip) c += 2 * __popcll(L.plane[1][lane]) + 4 * __popcll(L.plane[2][lane]);
+      c = __popcll(Q.plane[0][lane]);
+      if (clip) c += 2 * __popcll(Q.plane[1][lane]) + 4 * __popcll(Q.plane[2][lane]);
     }
     const int ic = wave_scan_add(c);
-    if (lane < R) L.rowstart[lane] = (uint32_t)(ic - c);
-    if (lane == R - 1) L.rowstart[R] = (uint32_t)ic;
+    if (lane < R) Q.rowstart[lane] = (uint32_t)(ic - c);
+    if (lane == R - 1) Q.rowstart[R] = (uint32_t)ic;
   }
   wave_sync();
 
@@ -524,10 +532,10 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
     if (!(prb[it] >> 16)) continue;
     const int r = prb[it] & 63, b = (prb[it] >> 8) & 0xff;
     const unsigned long long below = (1ull << r) - 1ull;
-    uint32_t pos = L.rowstart[b] + (uint32_t)__popcll(L.plane[0][b] & below);
+    uint32_t pos = Q.rowstart[b] + (uint32_t)__popcll(Q.plane[0][b] & below);
     if (clip)
-      pos += 2u * (uint32_t)__popcll(L.plane[1][b] & below) +
-             4u * (uint32_t)__popcll(L.plane[2][b] & below);
+      pos += 2u * (uint32_t)__popcll(Q.plane[1][b] & below) +
+             4u * (uint32_t)__popcll(Q.plane[2][b] & below);
     uint32_t colw = 0u;
     if constexpr (OM == 0) colw = L.col[r];
     const RowRec& Rr = L.rec[r];
@@ -540,13 +548,13 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
       fetch_slot(D, s, Rr.A[0], (uint32_t)Rr.A[1], cv, rgba, flat);
       const int c = clipped_spans(cv, F, jb0 + b, true, lo, hi, tz, zx, qs);
       for (int k = 0; k < c && pos + (uint32_t)k < (uint32_t)kCap; ++k)
-        L.u.ent[pos + k] = make_uint4(
+        Q.u.ent[pos + k] = make_uint4(
             (uint32_t)lo[k] | ((uint32_t)hi[k] << 8) |
                 ((((uint32_t)r << 3) | (uint32_t)qs[k]) << 16),
             __float_as_uint(tz[k]), __float_as_uint(zx[k]), colw);
     } else if (pos < (uint32_t)kCap) {
       const float tz = fmaf(Rr.zy, (float)(jb0 + b) + 0.5f, Rr.zc);
-      L.u.ent[pos] = make_uint4((uint32_t)plh[it] | ((uint32_t)r << 19), __float_as_uint(tz),
+      Q.u.ent[pos] = make_uint4((uint32_t)plh[it] | ((uint32_t)r << 19), __float_as_uint(tz),
                                 __float_as_uint(Rr.zx), colw);
     }
   }
@@ -727,10 +735,12 @@ __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const Scen
   }
 }
 
+// GL rows [ja, jb) of scene s, lists in Q.
 template <bool SMOOTH, int OM, int PXL, bool REF>
 __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_frame_out& O,
                                              const RowsFrame& F, const Light& lt, int s,
-                                             RowsLds<SMOOTH, OM>& L, int lane) {
+                                             const RowsLds<SMOOTH, OM>& L, ListLds& Q, int lane,
+                                             int ja, int jb) {
   constexpr bool OM0 = OM == 0;
   const int nrec = __builtin_amdgcn_readfirstlane(L.nrec);
   int rj0 = 1, rj1 = 0, rw = 1;
@@ -750,9 +760,9 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
   so.vis = O.vis ? O.vis + (size_t)s * so.plane : nullptr;
   // lane -> (row q of a group of four, pixels x0 .. x0 + PXL - 1)
   const int q = lane >> 4;
-  for (int jb0 = 0; jb0 < H;) {
+  for (int jb0 = ja; jb0 < jb;) {
     // band: at most kBandRows rows whose entries fit the list
-    int R = min(kBandRows, H - jb0);
+    int R = min(kBandRows, jb - jb0);
     int E;
     for (;;) {
       const int lo_row = max(rj0, jb0), hi_row = min(rj1, jb0 + R - 1);
@@ -762,7 +772,7 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
       R = (R + 1) >> 1;
     }
     if (E <= F.cap) {
-      build_lists<SMOOTH, OM>(D, F, s, L, lane, jb0, R, 0, nrec, rj0, rj1);
+      build_lists<SMOOTH, OM>(D, F, s, L, Q, lane, jb0, R, 0, nrec, rj0, rj1);
       // per-lane constants of the row walk, made after the lists so they are
       // not held through them
       const int x0 = (lane & 15) * PXL;
@@ -776,8 +786,8 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
         const bool row_ok = b < R;
         uint32_t n0 = 0, n1 = 0;
         if (row_ok) {
-          n0 = L.rowstart[b];
-          n1 = L.rowstart[b + 1];
+          n0 = Q.rowstart[b];
+          n1 = Q.rowstart[b + 1];
         }
         const int cnt = (int)(n1 - n0);
         const int emax =
@@ -785,7 +795,7 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
                 max(__builtin_amdgcn_readlane(cnt, 32), __builtin_amdgcn_readlane(cnt, 48)));
         PixState<PXL> S;
         init_state<PXL>(S, bg);
-        walk_row<PXL, OM0, REF>(L.u.ent, n0, cnt, emax, x0, pxf, S);
+        walk_row<PXL, OM0, REF>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
         if (row_ok) resolve_lane<SMOOTH, OM, PXL>(D, so, F, lt, s, L, x0, jb0 + b, S);
       }
       wave_sync();
@@ -810,11 +820,11 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
         if (!m) continue;
         const int r0 = __ffsll((unsigned long long)m) - 1;
         const int r1 = 64 - __clzll((long long)m);
-        build_lists<SMOOTH, OM>(D, F, s, L, lane, j, 1, r0, r1, rj0, rj1);
-        const uint32_t n0 = L.rowstart[0];
-        const int cnt = q == 0 ? (int)(L.rowstart[1] - n0) : 0;
+        build_lists<SMOOTH, OM>(D, F, s, L, Q, lane, j, 1, r0, r1, rj0, rj1);
+        const uint32_t n0 = Q.rowstart[0];
+        const int cnt = q == 0 ? (int)(Q.rowstart[1] - n0) : 0;
         const int emax = __builtin_amdgcn_readlane(cnt, 0);
-        walk_row<PXL, OM0, REF>(L.u.ent, n0, cnt, emax, x0, pxf, S);
+        walk_row<PXL, OM0, REF>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
         wave_sync();
       }
       if (q == 0) resolve_lane<SMOOTH, OM, PXL>(D, so, F, lt, s, L, x0, j, S);
@@ -827,13 +837,14 @@ template <bool SMOOTH, int OM, int PXL, bool REF>
 __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, pbr_frame_out O,
                                                              RowsFrame F, Light lt, float4 bg) {
   __shared__ RowsLds<SMOOTH, OM> lds[2];
+  __shared__ ListLds lists[kRowsBlock / 64];
   const int tid = (int)threadIdx.x;
   // the wave index is uniform within the wave: keep it (and what derives
   // from it - scene, LDS region, output bases) in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int s0 = (int)blockIdx.x * 2;
-  if (lane == 0) lds[wave].any_clipped = 0;
-  if (lane == 1) {
+  if (wave < 2 && lane == 0) lds[wave].any_clipped = 0;
+  if (wave < 2 && lane == 1) {
     const float bgc[4] = {bg.x, bg.y, bg.z, bg.w};
     lds[wave].col[kBg] = pack_col<OM>(bgc);
   }
@@ -851,9 +862,13 @@ __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, p
     }
   }
   __syncthreads();
-  const int s = s0 + wave;
+  // kWavesPerScene consecutive waves share scene s0 + w, each taking an
+  // equal share of its rows
+  const int sl = wave / kWavesPerScene, part = wave % kWavesPerScene;
+  const int s = s0 + sl;
   if (s >= D.num_scenes) return;
-  raster_scene<SMOOTH, OM, PXL, REF>(D, O, F, lt, s, lds[wave], lane);
+  const int ja = (F.H * part) / kWavesPerScene, jb = (F.H * (part + 1)) / kWavesPerScene;
+  raster_scene<SMOOTH, OM, PXL, REF>(D, O, F, lt, s, lds[sl], lists[wave], lane, ja, jb);
 }
 
 template <bool SMOOTH, int OM, bool REF>
