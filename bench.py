@@ -3,7 +3,8 @@
 Workload: the 64x64 CartPole scene with cart, pole and a shared ground quad
 (CartPoleGroundRenderer), 4096 scenes, uint8 RGB output [4096, 3, 64, 64].
 A step is one renderer.step(state): state [4096, 4] -> cart/pole instance
-transforms on the device (one pose launch) -> setup kernel -> raster kernel.
+transforms on the device (one pose launch) -> one row-span render launch
+(setup, row lists and pixel walk of every scene).
 States are seeded (x ~ U(-2.4, 2.4), theta ~ U(-0.21, 0.21)), one batch per
 step, and resident on the GPU before timing starts.
 
@@ -12,7 +13,7 @@ Printed JSON line (one, from this single process):
                (synchronize on both sides); `graphed_step` is the same step
                captured once as a hipGraph and replayed (it adds the copy of
                each state batch into the graph's static input)
-  roofline     the render call (setup kernel + raster kernel): algorithmic
+  roofline     the render call (one row-span kernel launch): algorithmic
                bytes per call (uint8 RGB pixels + 80 B per dynamic instance +
                64 B VP per scene) / average time of the call from HIP events
                around back-to-back calls on the launch stream, vs 8 TB/s
@@ -272,16 +273,38 @@ def env_loop_leg(n: int = 16384, steps: int = 1000) -> dict:
     }
 
 
-def measured_traffic() -> int | None:
+def render_kernel_name(r) -> str:
+    """The kernel family the library's AUTO choice runs for this renderer
+    (the rule of choose_kernel in csrc/pbr_render.hip)."""
+    if r.cfg.kernel != "auto":
+        return r.cfg.kernel
+    slots = sum(n.instances_per_scene * n.mesh.num_triangles for n in r._drawn_nodes())
+    W = int(r.cfg.tile_resolution[0])
+    if not r.cfg.persistent and slots <= 64 and W <= 128:
+        return "rows"
+    return "scene" if not r.cfg.persistent and slots <= 64 else "tiles"
+
+
+KERNEL_LABEL = {"rows": "rows_kernel (setup + lists + row walk, one launch)",
+                "scene": "setup_kernel + raster_kernel",
+                "tiles": "tile_prep_kernel + tile_setup_kernel + tile_raster_kernel",
+                "chunked": "chunked_kernel"}
+
+
+def measured_traffic(kernel: str) -> int | None:
     """HBM bytes per render call of this workload from the committed counter
-    profile (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
+    profile of the same kernel (FETCH_SIZE and WRITE_SIZE scaled by a copy of
+    known size)."""
     prof = ROOT / "profiles" / "render_cartpole_ground_4096x64x64_u8.md"
     if not prof.exists():
         return None
+    traffic, prof_kernel = None, None
     for line in prof.read_text().splitlines():
         if line.startswith("traffic_bytes_per_call:"):
-            return int(line.split(":")[1].strip())
-    return None
+            traffic = int(line.split(":")[1].strip())
+        if line.startswith("kernel_family:"):
+            prof_kernel = line.split(":")[1].strip()
+    return traffic if prof_kernel == kernel else None
 
 
 def copy_calibration(mib: int = 256, reps: int = 5) -> None:
@@ -440,8 +463,8 @@ def main() -> int:
                    "tris_per_scene": 26, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(),
-                     "kernel": "setup_kernel<false> + raster_kernel<8,4,false,0>", "kernel_us": round(kus, 3),
+                     "traffic": measured_traffic(render_kernel_name(r)),
+                     "kernel": KERNEL_LABEL[render_kernel_name(r)], "kernel_us": round(kus, 3),
                      "bytes_per_launch": nbytes},
     }
     rec["graphed_step"] = {"scene_frames_per_s": round(graphed, 1),

@@ -259,25 +259,48 @@ struct FixTri {
   float invw[3];
 };
 
-// (b) snap, cull, bound, fill-rule bias and depth plane of one clip-space
-// triangle; false if it yields no fragment in GL rows [jr0, jr1].
-__device__ __forceinline__ bool setup_fixed(const CVert& v0, const CVert& v1, const CVert& v2,
-                                            int W, int H, int jr0, int jr1, FixTri& T) {
-  const CVert* v[3] = {&v0, &v1, &v2};
+// One vertex of (b): perspective divide, window position snapped to 1/256
+// pixel, window depth and 1/w.  ok = false when w <= 0 or the position is
+// not finite (the triangle is then dropped).
+struct SnapVert {
+  int32_t X, Y;
+  float d, invw;
+  bool ok;
+};
+
+__device__ __forceinline__ void snap_vertex(const CVert& v, int W, int H, SnapVert& S) {
+  S.ok = false;
+  S.X = S.Y = 0;
+  S.d = S.invw = 0.0f;
+  float w = v.p[3];
+  if (!(w > 0.0f)) return;
+  float nx = v.p[0] / w, ny = v.p[1] / w, nz = v.p[2] / w;
+  float sx = (nx + 1.0f) * (0.5f * (float)W);
+  float sy = (ny + 1.0f) * (0.5f * (float)H);
+  if (!(fabsf(sx) < 65536.0f) || !(fabsf(sy) < 65536.0f)) return;  // non-finite input
+  S.X = (int32_t)rintf(sx * 256.0f);
+  S.Y = (int32_t)rintf(sy * 256.0f);
+  S.d = nz * 0.5f + 0.5f;
+  S.invw = 1.0f / w;
+  S.ok = true;
+}
+
+// The per-triangle rest of (b) on three snapped vertices: cull, bound,
+// fill-rule bias and depth plane; false if the triangle yields no fragment
+// in GL rows [jr0, jr1].
+__device__ __forceinline__ bool setup_snapped(const SnapVert& s0, const SnapVert& s1,
+                                              const SnapVert& s2, int W, int H, int jr0, int jr1,
+                                              FixTri& T) {
+  if (!(s0.ok && s1.ok && s2.ok)) return false;
+  const SnapVert* sv[3] = {&s0, &s1, &s2};
   int64_t X[3], Y[3];
   float d[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float w = v[k]->p[3];
-    if (!(w > 0.0f)) return false;
-    float nx = v[k]->p[0] / w, ny = v[k]->p[1] / w, nz = v[k]->p[2] / w;
-    float sx = (nx + 1.0f) * (0.5f * (float)W);
-    float sy = (ny + 1.0f) * (0.5f * (float)H);
-    if (!(fabsf(sx) < 65536.0f) || !(fabsf(sy) < 65536.0f)) return false;  // non-finite input
-    X[k] = (int64_t)rintf(sx * 256.0f);
-    Y[k] = (int64_t)rintf(sy * 256.0f);
-    d[k] = nz * 0.5f + 0.5f;
-    T.invw[k] = 1.0f / w;
+    X[k] = sv[k]->X;
+    Y[k] = sv[k]->Y;
+    d[k] = sv[k]->d;
+    T.invw[k] = sv[k]->invw;
   }
   int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
   if (area <= 0) return false;
@@ -318,6 +341,17 @@ __device__ __forceinline__ bool setup_fixed(const CVert& v0, const CVert& v1, co
   T.zy = (float)zyd;
   T.zc = (float)zcd;
   return true;
+}
+
+// (b) snap, cull, bound, fill-rule bias and depth plane of one clip-space
+// triangle; false if it yields no fragment in GL rows [jr0, jr1].
+__device__ __forceinline__ bool setup_fixed(const CVert& v0, const CVert& v1, const CVert& v2,
+                                            int W, int H, int jr0, int jr1, FixTri& T) {
+  SnapVert s0, s1, s2;
+  snap_vertex(v0, W, H, s0);
+  snap_vertex(v1, W, H, s1);
+  snap_vertex(v2, W, H, s2);
+  return setup_snapped(s0, s1, s2, W, H, jr0, jr1, T);
 }
 
 // Exact edge functions e_k = A*Px + B*Py + C of a snapped triangle (edge k is

@@ -59,6 +59,7 @@ struct RowsFrame {
   float guard;
   int seg;  // setup lanes per scene (power of two >= slots, <= 64)
   int cap;  // row-list entries per band (8..kCap)
+  int vphase;  // 1: vertices transformed once into an LDS table (they fit)
 };
 
 // Unclipped: edge coefficients, depth plane and rows of a snapped triangle.
@@ -408,6 +409,211 @@ __device__ __forceinline__ void setup_slot(const pbr_scene_desc& D, const RowsFr
 #pragma unroll
     for (int c = 0; c < 4; ++c) sh[12 + c] = rgba[c];
   }
+}
+
+// Vertex table entry: one (instance, mesh vertex) of a scene after (a) and
+// the per-vertex part of (b) - each is transformed and snapped once, however
+// many triangles share it.
+struct VertRec {
+  int32_t X, Y;
+  float d, invw;
+  float n[3];     // normalised world normal
+  uint32_t bits;  // bit 0: snapped; bit 1 + p: inside plane p; bit 7 + p: outside plane p
+};
+static_assert(sizeof(VertRec) == 32, "VertRec is 32 bytes");
+
+// The vertex table of a workgroup's two scenes lives in the (not yet used)
+// row-list space.
+constexpr int kVertCap = (int)((kRowsBlock / 64) * sizeof(ListLds) / sizeof(VertRec));
+
+// (a) + per-vertex (b) of vertex slot v of scene s (draw order: node,
+// instance, mesh vertex).
+__device__ __forceinline__ void vertex_slot(const pbr_scene_desc& D, const RowsFrame& F, int s,
+                                            uint32_t v, VertRec& out) {
+  const float* trans = nullptr;
+  int ips = 1, shared = 0, voff = 0, nv = 1;
+  uint32_t local = 0, acc = 0;
+  bool found = false;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    const pbr_node& nd = D.nodes[j];
+    const pbr_mesh& me = D.meshes[nd.mesh];
+    int cnt = nd.instances_per_scene;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
+    const uint32_t n = (uint32_t)cnt * (uint32_t)me.num_vertices;
+    if (!found && v < acc + n) {
+      found = true;
+      local = v - acc;
+      trans = nd.transforms;
+      ips = nd.instances_per_scene;
+      shared = nd.shared;
+      voff = me.vertex_offset;
+      nv = me.num_vertices;
+    }
+    acc += n;
+  }
+  out.bits = 0u;
+  if (!found) return;
+  const uint32_t i = local / (uint32_t)nv, k = local - i * (uint32_t)nv;
+  const size_t row = shared ? (size_t)i : (size_t)s * (size_t)ips + (size_t)i;
+  const float4* M4 = reinterpret_cast<const float4*>(trans + row * 16);
+  const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2], m3 = M4[3];
+  const float M[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
+                       m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+  const float4* V4 = reinterpret_cast<const float4*>(D.view_proj + (size_t)s * 16);
+  const float4 q0 = V4[0], q1 = V4[1], q2 = V4[2], q3 = V4[3];
+  const float VP[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                        q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const float* vv = D.vertices + (size_t)(voff + (int)k) * 6;
+  CVert cv;
+  float wp[4], nw[3];
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    float a = M[rr * 4 + 0] * vv[0] + M[rr * 4 + 1] * vv[1];
+    a = a + M[rr * 4 + 2] * vv[2];
+    wp[rr] = a + M[rr * 4 + 3];
+  }
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    float a = VP[rr * 4 + 0] * wp[0] + VP[rr * 4 + 1] * wp[1];
+    a = a + VP[rr * 4 + 2] * wp[2];
+    cv.p[rr] = a + VP[rr * 4 + 3] * wp[3];
+  }
+#pragma unroll
+  for (int rr = 0; rr < 3; ++rr) {
+    float a = M[rr * 4 + 0] * vv[3] + M[rr * 4 + 1] * vv[4];
+    nw[rr] = a + M[rr * 4 + 2] * vv[5];
+  }
+  normalize3(nw, cv.n);
+  SnapVert sv;
+  snap_vertex(cv, F.W, F.H, sv);
+  uint32_t bits = sv.ok ? 1u : 0u;
+#pragma unroll
+  for (int pl = 0; pl < 6; ++pl) {
+    const float dpl = plane_dist(cv, pl, F.guard);
+    if (dpl >= 0.0f) bits |= 2u << pl;
+    if (dpl < 0.0f) bits |= 128u << pl;
+  }
+  out.X = sv.X;
+  out.Y = sv.Y;
+  out.d = sv.d;
+  out.invw = sv.invw;
+  out.n[0] = cv.n[0];
+  out.n[1] = cv.n[1];
+  out.n[2] = cv.n[2];
+  out.bits = bits;
+}
+
+// (a)+(b) of one slot of scene s from the scene's vertex table `vt`: the
+// same record setup_slot makes, with the vertices read back instead of
+// transformed again.
+template <bool SMOOTH, int OM>
+__device__ __forceinline__ void setup_slot_v(const pbr_scene_desc& D, const RowsFrame& F,
+                                             const Light& lt, int s, uint32_t slot,
+                                             RowsLds<SMOOTH, OM>& L, const VertRec* vt) {
+  uint32_t local, total;
+  const int node = locate_slot(D, s, slot, local, total);
+  if (slot == 0) L.nrec = (int)total;
+  if (node < 0) return;
+  const float* cols = nullptr;
+  int ips = 1, shared = 0, ioff = 0, ntri = 1, nv = 1, fl = 0;
+  uint32_t vbase = 0, vacc = 0;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    const pbr_node& nd = D.nodes[j];
+    const pbr_mesh& me = D.meshes[nd.mesh];
+    if (node == j) {
+      cols = nd.colors;
+      ips = nd.instances_per_scene;
+      shared = nd.shared;
+      fl = nd.flat;
+      ioff = me.index_offset;
+      ntri = me.num_triangles;
+      nv = me.num_vertices;
+      vbase = vacc;
+    }
+    int cnt = nd.instances_per_scene;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
+    vacc += (uint32_t)cnt * (uint32_t)me.num_vertices;
+  }
+  const uint32_t i = local / (uint32_t)ntri, t = local - i * (uint32_t)ntri;
+  const size_t row = shared ? (size_t)i : (size_t)s * (size_t)ips + (size_t)i;
+  const int32_t* tri = D.indices + (size_t)(ioff + (int)t) * 3;
+  const uint32_t vb = vbase + i * (uint32_t)nv;
+  const VertRec a = vt[vb + (uint32_t)tri[0]];
+  const VertRec b = vt[vb + (uint32_t)tri[1]];
+  const VertRec c = vt[vb + (uint32_t)tri[2]];
+  const float4 c4 = reinterpret_cast<const float4*>(cols)[row];
+  const float rgba[4] = {c4.x, c4.y, c4.z, c4.w};
+  const bool flat = fl != 0;
+  const uint32_t inb = (a.bits & b.bits & c.bits);
+  const bool all_in = ((inb >> 1) & 0x3fu) == 0x3fu;
+  const bool all_out = ((inb >> 7) & 0x3fu) != 0u;
+  RowRec R;
+  R.j0 = 1;
+  R.j1 = 0;
+  uint32_t rfl = 0;
+  const bool smooth = SMOOTH && !flat;
+  FixTri T;
+  bool have_T = false;
+  if (!all_out) {
+    if (all_in) {
+      const SnapVert sa = {a.X, a.Y, a.d, a.invw, (a.bits & 1u) != 0u};
+      const SnapVert sb = {b.X, b.Y, b.d, b.invw, (b.bits & 1u) != 0u};
+      const SnapVert sc = {c.X, c.Y, c.d, c.invw, (c.bits & 1u) != 0u};
+      if (setup_snapped(sa, sb, sc, F.W, F.H, 0, F.H - 1, T)) {
+        edges_of(T, R);
+        R.j0 = (int16_t)T.j0;
+        R.j1 = (int16_t)T.j1;
+        rfl = T.bias;
+        have_T = true;
+      }
+    } else {
+      CVert cv[3];
+      float rgba2[4];
+      bool flat2;
+      fetch_slot(D, s, node, local, cv, rgba2, flat2);
+      int j0, j1;
+      const int n = clipped_rows(cv, F, j0, j1);
+      if (j0 <= j1) {
+        R.A[0] = node;
+        R.A[1] = (int32_t)local;
+        R.j0 = (int16_t)j0;
+        R.j1 = (int16_t)j1;
+        rfl = kRFlagClipped | ((uint32_t)(n - 2) << 5);
+        atomicOr(&L.any_clipped, 1);
+      }
+    }
+  }
+  if (smooth) rfl |= kRFlagSmooth;
+  L.rec[slot] = R;
+  L.flags[slot] = (uint8_t)rfl;
+  if (!smooth) {
+    float flat_rgba[4];
+    shade(a.n, rgba, lt, flat_rgba);
+    L.col[slot] = pack_col<OM>(flat_rgba);
+  } else if (SMOOTH && have_T) {
+    float* sh = &L.sh[slot * kShadeFloats];
+    const VertRec* vs[3] = {&a, &b, &c};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) sh[3 * k + q] = vs[k]->n[q];
+      sh[9 + k] = T.invw[k];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sh[12 + q] = rgba[q];
+  }
+}
+
+// Vertex slots of scene s: sum over nodes of active instances x mesh vertices.
+__device__ __forceinline__ uint32_t scene_vertex_slots(const pbr_scene_desc& D, int s) {
+  uint32_t total = 0;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    const pbr_node& nd = D.nodes[j];
+    int cnt = nd.instances_per_scene;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
+    total += (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_vertices;
+  }
+  return total;
 }
 
 // Colour of a smooth winner at pixel (x, j): per-pixel interpolation of the
@@ -849,7 +1055,27 @@ __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, p
     lds[wave].col[kBg] = pack_col<OM>(bgc);
   }
   __syncthreads();
-  {
+  if (F.vphase) {
+    // vertex phase: every thread takes vertex slots of the two scenes (the
+    // table sits in the row-list space, which is not in use yet)
+    VertRec* vt = reinterpret_cast<VertRec*>(&lists[0]);
+    const uint32_t nv0 = s0 < D.num_scenes ? scene_vertex_slots(D, s0) : 0u;
+    const uint32_t nv1 = s0 + 1 < D.num_scenes ? scene_vertex_slots(D, s0 + 1) : 0u;
+    for (uint32_t v = (uint32_t)tid; v < nv0 + nv1; v += (uint32_t)kRowsBlock) {
+      const int sv = v < nv0 ? 0 : 1;
+      vertex_slot(D, F, s0 + sv, sv ? v - nv0 : v, vt[v]);
+    }
+    __syncthreads();
+    const int sl = tid / F.seg;
+    const uint32_t slot = (uint32_t)(tid % F.seg);
+    const int s = s0 + sl;
+    if (sl < 2) {
+      if (s < D.num_scenes)
+        setup_slot_v<SMOOTH, OM>(D, F, lt, s, slot, lds[sl], vt + (sl ? nv0 : 0u));
+      else if (slot == 0)
+        lds[sl].nrec = 0;
+    }
+  } else {
     // setup lanes: tid -> (scene s0 + tid / seg, slot tid % seg)
     const int sl = tid / F.seg;
     const uint32_t slot = (uint32_t)(tid % F.seg);
@@ -910,6 +1136,12 @@ int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rende
   if (cap != 0 && (cap < 8 || cap > kCap))
     return fail(PBR_ERR_INVALID, "row_list_cap %d outside [8, %d]", cap, kCap);
   F.cap = cap ? cap : kCap;
+  // the vertex phase needs both scenes' vertex slots in the table
+  uint64_t vslots = 0;
+  for (int j = 0; j < D->num_nodes; ++j)
+    vslots += (uint64_t)D->nodes[j].instances_per_scene *
+              (uint64_t)D->meshes[D->nodes[j].mesh].num_vertices;
+  F.vphase = (2 * vslots <= (uint64_t)kVertCap) ? 1 : 0;
   const Light lt = host_light(D);
   const float4 bg = make_float4(D->background[0], D->background[1], D->background[2],
                                 D->background[3]);

@@ -402,6 +402,25 @@ def test_rows_kernel_smooth_through_the_near_plane(cuda):
     check_parity(r, "rows kernel smooth near-plane clipping")
 
 
+def test_rows_kernel_per_triangle_setup(cuda):
+    """Meshes with many more vertices than their triangles use (here 400
+    vertices, 12 triangles) overflow the vertex table; such batches set up
+    each triangle on its own lane."""
+    from pybatchrender_amd.renderer import mesh
+
+    box = mesh.box()
+    pad = 400 - box.num_vertices
+    rng = np.random.default_rng(38)
+    pos = np.concatenate([box.positions, rng.uniform(-1, 1, (pad, 3))])
+    nrm = np.concatenate([box.normals, np.tile([[0.0, 0.0, 1.0]], (pad, 1))])
+    tris = box.triangles + 0
+    order = rng.permutation(len(pos))  # scatter the used vertices through the table
+    inv = np.argsort(order)
+    sparse = mesh.Mesh(pos[order], nrm[order], inv[tris], name="sparse_box")
+    r = _small_scene(10, (64, 64), 38, model=sparse, instances=3, kernel="rows")
+    check_parity(r, "rows kernel per-triangle setup")
+
+
 def test_auto_beyond_rows_limits(cuda):
     """Tiles wider than 128 pixels: AUTO takes the scene kernel instead."""
     r = _small_scene(6, (130, 40), 37)
