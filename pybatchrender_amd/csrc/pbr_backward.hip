@@ -33,7 +33,16 @@
 //            light and vertex gradients (LDS tables);
 //   phase 4  the workgroup writes its light / vertex / shared-colour partial
 //            row; reduce_kernel sums the rows of all workgroups.
+//
+// Scenes whose nodes all shade flat and whose slots fit one LDS pass take
+// backward_flat_kernel instead: a persistent grid of five-wave workgroups in
+// which four waves stream the pixel runs of one scene while the fifth runs
+// the per-slot chain of the previous one (double-buffered accumulators), so
+// the chain's dependent loads hide behind the pixel stream; one partial row
+// per workgroup.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -80,6 +89,7 @@ struct BwdFrame {
   int nwords;      // bitmap words (compact)
   int vtab;        // vertex table entries in LDS (0: global atomics)
   int stab;        // shared instance rows in LDS (0: global atomics)
+  int ttab;        // flat kernel: triangles whose first vertex an LDS table holds
   int P;           // floats per workgroup partial row
   int off_v, off_s;  // partial row: vertex block, shared-colour block
   int soff[PBR_MAX_NODES];  // first shared row of node j, -1 if not shared
@@ -298,6 +308,26 @@ __device__ __forceinline__ int pass_index(const BwdFrame& F, const BwdLds& S, ui
   }
   ci -= pbase;
   return (ci >= 0 && ci < cnt) ? ci : -1;
+}
+
+// The light, vertex and shared-colour sums of one workgroup as its partial
+// row (layout: light[10], then F.off_v, then F.off_s).
+__device__ __forceinline__ void write_partial(const BwdFrame& F, const BwdLight& L,
+                                              const float* light, const float* vtab,
+                                              const float* stab, float* row, int tid,
+                                              int nthreads) {
+  if (tid == 0) {
+    // d/d light_dir from d/d L^: (g - L^ (L^ . g)) / |light_dir|
+    const float g0 = light[0], g1 = light[1], g2 = light[2];
+    const float d = L.dir[0] * g0 + L.dir[1] * g1 + L.dir[2] * g2;
+    row[0] = (g0 - L.dir[0] * d) * L.inv_len;
+    row[1] = (g1 - L.dir[1] * d) * L.inv_len;
+    row[2] = (g2 - L.dir[2] * d) * L.inv_len;
+    for (int q = 3; q < 9; ++q) row[q] = light[q];
+    row[9] = light[9] * L.s_pass;
+  }
+  for (int i = tid; i < F.vtab * 6; i += nthreads) row[F.off_v + i] = vtab[i];
+  for (int i = tid; i < F.stab * 4; i += nthreads) row[F.off_s + i] = stab[i];
 }
 
 // SMOOTH: some node shades with interpolated normals (32 accumulators per
@@ -696,26 +726,380 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
   }
 
   // phase 4: this workgroup's partial row
-  float* row = partial + (size_t)s * F.P;
-  if (tid == 0) {
-    // d/d light_dir from d/d L^: (g - L^ (L^ . g)) / |light_dir|
-    const float g0 = S.light[0], g1 = S.light[1], g2 = S.light[2];
-    const float d = L.dir[0] * g0 + L.dir[1] * g1 + L.dir[2] * g2;
-    row[0] = (g0 - L.dir[0] * d) * L.inv_len;
-    row[1] = (g1 - L.dir[1] * d) * L.inv_len;
-    row[2] = (g2 - L.dir[2] * d) * L.inv_len;
-    for (int q = 3; q < 9; ++q) row[q] = S.light[q];
-    row[9] = S.light[9] * L.s_pass;
+  write_partial(F, L, S.light, S.vtab, S.stab, partial + (size_t)s * F.P, tid, kBwdBlock);
+}
+
+// --- flat scenes: persistent, wave-specialised workgroups --------------------
+// When every node shades flat and the slots fit one LDS pass, a slot's
+// gradient depends on the pixels only through the per-channel sum of the
+// output gradient over its pixels.  A workgroup then walks the scenes
+// blockIdx.x, blockIdx.x + gridDim.x, ...: in step k waves 1-4 sum the runs
+// of scene k into one half of a double-buffered accumulator table while
+// wave 0 runs the chain of scene k - 1 from the other half and clears it.
+// The chain's dependent loads (instance row, indices, then the vertex) thus
+// overlap the next scene's pixel stream instead of stalling the workgroup,
+// and each workgroup writes one partial row for all of its scenes.
+constexpr int kFlatBlock = 320;
+constexpr int kFlatPix = kFlatBlock - 64;
+constexpr int kMaxFlatGrid = 2048;
+constexpr int kMaxFlatTtab = 2048;  // triangles of the first-vertex table (32 KiB)
+
+// Sum of v over the 64 lanes, in lane 63 (every lane active): row-wise
+// inclusive scans by row shifts 1, 2, 4, 8, then row broadcasts of lanes 15
+// and 31; lanes without a source add 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_src(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum63(float v) {
+  v += dpp_src<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_src<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_src<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_src<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_src<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_src<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+// Waves 1-4: the runs of one scene into acc[cap][4] (sums of g_c per slot).
+// A run is 16 pixels down one column; run r covers column r % W of the band
+// of rows [16 (r / W), +16), so the 64 lanes of a wave read consecutive
+// columns of one row per step (coalesced) and walk across horizontal edges
+// only.  Pixels whose id is not a slot of the table (background, out of
+// range) are summed into a discarded run.  The loop over runs is
+// wave-uniform (lanes past the last run walk an empty one), so the
+// end-of-run flush can combine lanes: the slots most lanes end on (the large
+// triangles of the scene, whose lanes would otherwise queue on one LDS
+// address) are summed across the wave first and added by one lane; the rest
+// add their own sums.
+template <int NC>
+__device__ __forceinline__ void flat_runs(const BwdFrame& F, const int32_t* vscene,
+                                          const float* gscene, size_t plane, float* acc, int p) {
+  const int W = F.W, H = F.H;
+  const int bands = (H + kRun - 1) / kRun;
+  const int nruns = bands * W;
+  const uint32_t cap = (uint32_t)F.cap;
+  const int lane = p & 63;
+  for (int rb = p & ~63; rb < nruns; rb += kFlatPix) {
+    const int run = rb + lane;
+    const bool live = run < nruns;
+    const int band = live ? run / W : 0;
+    const int x = live ? run - band * W : 0;
+    const int y0 = band * kRun;
+    const int n = live ? min(kRun, H - y0) : 0;
+    const int32_t* vc = vscene + (size_t)y0 * W + x;
+    const float* gc = gscene + (size_t)y0 * W + x;
+    int32_t vv[kRun];
+    float gg[NC][kRun];
+    // every load is issued (rows past the band's end re-read its last row, an
+    // empty run reads row 0 of the scene) and masked afterwards
+    const int last = max(n - 1, 0);
+#pragma unroll
+    for (int i = 0; i < kRun; ++i) {
+      const size_t off = (size_t)min(i, last) * W;
+      vv[i] = vc[off];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) gg[c][i] = gc[(size_t)c * plane + off];
+    }
+#pragma unroll
+    for (int i = 0; i < kRun; ++i) {
+      if (i >= n) vv[i] = -1;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) gg[c][i] = i < n ? gg[c][i] : 0.0f;
+    }
+    int cur = -1;
+    float a[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a[c] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kRun; ++i) {
+      const int li = (uint32_t)vv[i] < cap ? vv[i] : -1;
+      if (li != cur) {
+        if (cur >= 0) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) atomicAdd(acc + (size_t)cur * 4 + c, a[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a[c] = 0.0f;
+        cur = li;
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a[c] += gg[c][i];
+    }
+    // end of run: two wave-wide rounds for the most common open slots
+#pragma unroll
+    for (int round = 0; round < 2; ++round) {
+      const uint64_t open = __ballot(cur >= 0);
+      if (open == 0) break;
+      const int lead = __builtin_amdgcn_readlane(cur, __ffsll((unsigned long long)open) - 1);
+      const bool mine = cur == lead;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const float t = wave_sum63(mine ? a[c] : 0.0f);
+        if (lane == 63) atomicAdd(acc + (size_t)lead * 4 + c, t);
+      }
+      if (mine) cur = -1;
+    }
+    if (cur >= 0) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) atomicAdd(acc + (size_t)cur * 4 + c, a[c]);
+    }
   }
-  for (int i = tid; i < F.vtab * 6; i += kBwdBlock) row[F.off_v + i] = S.vtab[i];
-  for (int i = tid; i < F.stab * 4; i += kBwdBlock) row[F.off_s + i] = S.stab[i];
+}
+
+// Node fields the chain needs, one LDS entry per node (built once per
+// workgroup, so a slot's node costs LDS reads instead of dependent scalar
+// loads per scene).
+struct FlatNode {
+  const float* trans;
+  const float* cols;
+  float* gcol;
+  const int32_t* counts;
+  int32_t ips, ntri, shared, ioff, voff, soff, pad[2];
+};
+static_assert(sizeof(FlatNode) == 64, "FlatNode layout");
+
+// LDS of the flat kernel: float offsets of its tables.
+struct FlatLds {
+  float* acc;        // [2][cap][4] per-slot sums of g_c, double-buffered
+  float* light;      // [16]
+  FlatNode* nodes;   // [PBR_MAX_NODES]
+  int4* ttab;        // [ttab] first vertex of triangle q: id, normal bits
+  uint32_t* tn;      // [PBR_MAX_NODES] slots of node j in the current scene
+  float* glane;      // [kLightF][64] light gradient of wave 0's lanes
+  float* vtab;       // [vtab][6]
+  float* stab;       // [stab][4]
+};
+
+__device__ __forceinline__ FlatLds flat_lds(const BwdFrame& F, unsigned char* smem) {
+  FlatLds S;
+  float* p = reinterpret_cast<float*>(smem);
+  S.acc = p;
+  p += (size_t)F.cap * 8;
+  S.light = p;
+  p += 16;
+  S.nodes = reinterpret_cast<FlatNode*>(p);
+  p += PBR_MAX_NODES * 16;
+  S.ttab = reinterpret_cast<int4*>(p);
+  p += (size_t)F.ttab * 4;
+  S.tn = reinterpret_cast<uint32_t*>(p);
+  p += PBR_MAX_NODES;
+  S.glane = p;
+  p += kLightF * 64;
+  S.vtab = p;
+  p += (size_t)F.vtab * 6;
+  S.stab = p;
+  return S;
+}
+
+// Float atomic add through a global-address-space pointer (a generic
+// pointer would take the flat atomic's shared / private aperture checks).
+typedef __attribute__((address_space(1))) float global_float;
+typedef __attribute__((address_space(1))) const float global_cfloat;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f32x4 global_cfloat4;
+typedef __attribute__((address_space(1))) const int32_t global_cint;
+__device__ __forceinline__ void global_add(float* p, float v) {
+  __hip_atomic_fetch_add((global_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave 0: the per-slot chain of scene s from acc[cap][4], which it clears.
+// Lane-per-slot.  A flat slot shades with the normal of its first vertex, so
+// only that vertex's normal receives a gradient.  The light gradient stays in
+// the lane's own LDS column (S.glane) until the workgroup's last scene.
+__device__ __forceinline__ void flat_chain(const pbr_scene_desc& D, const BwdFrame& F,
+                                           const BwdLight& L, const FlatLds& S, int s,
+                                           float* acc, float* gverts, int lane) {
+  const int nn = D.num_nodes;
+  if (lane < nn) {
+    const FlatNode& nd = S.nodes[lane];
+    int cnt = nd.ips;
+    if (nd.counts) cnt = min(max(((global_cint*)nd.counts)[s], 0), nd.ips);
+    S.tn[lane] = (uint32_t)cnt * (uint32_t)nd.ntri;
+  }
+  wave_sync();
+  for (int li = lane; li < F.cap; li += 64) {
+    float4* ap = reinterpret_cast<float4*>(acc + (size_t)li * 4);
+    const float4 sum = *ap;
+    *ap = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (sum.x == 0.0f && sum.y == 0.0f && sum.z == 0.0f && sum.w == 0.0f) continue;
+    const uint32_t slot = (uint32_t)li;
+    int node = -1;
+    uint32_t base = 0, local = 0;
+    for (int j = 0; j < nn; ++j) {
+      const uint32_t n = S.tn[j];
+      if (node < 0 && slot < base + n) {
+        node = j;
+        local = slot - base;
+      }
+      base += n;
+    }
+    if (node < 0) continue;
+    const FlatNode& nd = S.nodes[node];
+    const uint32_t ntri = (uint32_t)nd.ntri;
+    const uint32_t i = local / ntri, t = local - i * ntri;
+    const size_t row = nd.shared ? (size_t)i : (size_t)s * (size_t)nd.ips + i;
+    global_cfloat4* M4 = (global_cfloat4*)(nd.trans + row * 16);
+    const f32x4 m0 = M4[0], m1 = M4[1], m2 = M4[2];
+    const float M[12] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
+    const f32x4 c4 = ((global_cfloat4*)nd.cols)[row];
+    const float rgba[4] = {c4.x, c4.y, c4.z, c4.w};
+    int vi;
+    float nv[3];
+    if (F.ttab > 0) {
+      const int4 e = S.ttab[nd.ioff + (int)t];
+      vi = e.x;
+      nv[0] = __int_as_float(e.y);
+      nv[1] = __int_as_float(e.z);
+      nv[2] = __int_as_float(e.w);
+    } else {
+      vi = nd.voff + ((global_cint*)D.indices)[(size_t)(nd.ioff + (int)t) * 3];
+      global_cfloat* vv = (global_cfloat*)D.vertices + (size_t)vi * 6;
+      nv[0] = vv[3];
+      nv[1] = vv[4];
+      nv[2] = vv[5];
+    }
+    float u[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      float a = M[r * 4 + 0] * nv[0] + M[r * 4 + 1] * nv[1];
+      u[r] = a + M[r * 4 + 2] * nv[2];
+    }
+    float nw[3];
+    normalize3(u, nw);
+    // every term is linear in the summed output gradient
+    const float gs[4] = {sum.x, sum.y, sum.z, sum.w};
+    float tacc[kAcc];
+#pragma unroll
+    for (int q = 0; q < kAcc; ++q) tacc[q] = 0.0f;
+    float gn[3];
+    shade_back(nw, rgba, gs, F.C, L, tacc, gn);
+    // instance colour
+    const float gc[4] = {tacc[A_COL], tacc[A_COL + 1], tacc[A_COL + 2], tacc[A_ALPHA]};
+    if (nd.shared && nd.soff >= 0) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) lds_add(S.stab + (size_t)(nd.soff + (int)row) * 4 + c, gc[c]);
+    } else if (nd.gcol) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (gc[c] != 0.0f) global_add(nd.gcol + row * 4 + c, gc[c]);
+    }
+#pragma unroll
+    for (int q = 0; q < kLightF; ++q) S.glane[q * 64 + lane] += tacc[A_GL + q];
+    // the first vertex's normal, through both normalisations
+    float gu[3];
+    normalize_back(nw, len3(u), gn, gu);
+    float gv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      gv[k] = M[0 * 4 + k] * gu[0] + M[1 * 4 + k] * gu[1] + M[2 * 4 + k] * gu[2];
+    if (F.vtab > 0) {
+      if ((unsigned)vi < (unsigned)F.vtab) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lds_add(S.vtab + (size_t)vi * 6 + 3 + k, gv[k]);
+      }
+    } else if (gverts) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (gv[k] != 0.0f) global_add(gverts + (size_t)vi * 6 + 3 + k, gv[k]);
+    }
+  }
+}
+
+// Wave 0, while waves 1-4 stream the first scene: the node table and the
+// first vertex (id, model normal) of every triangle of the index buffer.
+__device__ __forceinline__ void flat_tables(const pbr_scene_desc& D, const BwdFrame& F,
+                                            const BwdGrads& G, const FlatLds& S, int lane) {
+  if (lane == 0) {
+    for (int j = 0; j < D.num_nodes; ++j) {
+      const pbr_node& nd = D.nodes[j];
+      const pbr_mesh& me = D.meshes[nd.mesh];
+      FlatNode e;
+      e.trans = nd.transforms;
+      e.cols = nd.colors;
+      e.gcol = G.colors[j];
+      e.counts = nd.counts;
+      e.ips = nd.instances_per_scene;
+      e.ntri = me.num_triangles;
+      e.shared = nd.shared;
+      e.ioff = me.index_offset;
+      e.voff = me.vertex_offset;
+      e.soff = F.soff[j];
+      e.pad[0] = e.pad[1] = 0;
+      S.nodes[j] = e;
+    }
+  }
+  for (int q = lane; q < F.ttab; q += 64) {
+    int voff = 0;
+    bool found = false;
+    for (int m = 0; m < D.num_meshes; ++m) {
+      const pbr_mesh& me = D.meshes[m];
+      if (!found && q >= me.index_offset && q < me.index_offset + me.num_triangles) {
+        voff = me.vertex_offset;
+        found = true;
+      }
+    }
+    int4 e = make_int4(-1, 0, 0, 0);
+    if (found) {
+      const int vi = voff + ((global_cint*)D.indices)[(size_t)q * 3];
+      global_cfloat* vv = (global_cfloat*)D.vertices + (size_t)vi * 6;
+      e = make_int4(vi, __float_as_int(vv[3]), __float_as_int(vv[4]), __float_as_int(vv[5]));
+    }
+    S.ttab[q] = e;
+  }
+}
+
+template <int NC>
+__global__ __launch_bounds__(kFlatBlock, 5) void backward_flat_kernel(pbr_scene_desc D, BwdFrame F,
+                                                                   BwdLight L, BwdGrads G,
+                                                                   const int32_t* vis,
+                                                                   const float* gout,
+                                                                   float* partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const FlatLds S = flat_lds(F, smem);
+  const int tid = (int)threadIdx.x;
+  for (int i = tid; i < F.cap * 8 + 16; i += kFlatBlock) S.acc[i] = 0.0f;  // acc, light
+  for (int i = tid; i < kLightF * 64; i += kFlatBlock) S.glane[i] = 0.0f;
+  for (int i = tid; i < F.vtab * 6 + F.stab * 4; i += kFlatBlock) S.vtab[i] = 0.0f;
+  __syncthreads();
+  // wave-uniform role: every wave meets the same barriers below
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int b = (int)blockIdx.x, stride = (int)gridDim.x;
+  const int nk = (D.num_scenes - b + stride - 1) / stride;  // launch: gridDim.x <= num_scenes
+  const size_t plane = (size_t)F.W * (size_t)F.H;
+  for (int k = 0; k <= nk; ++k) {
+    if (wave == 0) {
+      if (k == 0) flat_tables(D, F, G, S, tid);
+      else
+        flat_chain(D, F, L, S, b + (k - 1) * stride, S.acc + (size_t)((k - 1) & 1) * F.cap * 4,
+                   G.vertices, tid);
+    } else if (k < nk) {
+      const int s = b + k * stride;
+      flat_runs<NC>(F, vis + (size_t)s * plane, gout + (size_t)s * NC * plane, plane,
+                    S.acc + (size_t)(k & 1) * F.cap * 4, tid - 64);
+    }
+    __syncthreads();
+  }
+  if (wave == 0) {
+#pragma unroll
+    for (int q = 0; q < kLightF; ++q) lds_add(S.light + q, S.glane[q * 64 + tid]);
+  }
+  __syncthreads();
+  write_partial(F, L, S.light, S.vtab, S.stab, partial + (size_t)b * F.P, tid, kFlatBlock);
 }
 
 // Sums the partial rows of `rows` workgroups: block (x, y) adds entries
-// [64x, 64x + 64) of rows [64y, 64y + 64) into the gradients.  Thread t
-// takes entry 64x + t % 64 of every fourth row (four loads in flight), the
-// four row lanes combine in LDS and one atomic per entry leaves the block.
-constexpr int kReduceRows = 64;
+// [64x, 64x + 64) of rows [32y, 32y + 32) into the gradients.  Thread t takes
+// entry 64x + t % 64 of eight rows, all eight loads in flight at once (rows
+// past the end re-read the last row and add nothing); the four row lanes
+// combine in LDS and one atomic per entry leaves the block.
+constexpr int kReduceRows = 32;
 
 __global__ __launch_bounds__(256) void reduce_kernel(BwdFrame F, BwdGrads G, int rows,
                                                      const float* partial, int nodes,
@@ -726,13 +1110,13 @@ __global__ __launch_bounds__(256) void reduce_kernel(BwdFrame F, BwdGrads G, int
   const int r0 = (int)blockIdx.y * kReduceRows, r1 = min(rows, r0 + kReduceRows);
   float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   if (e < F.P) {
-    for (int r = r0 + lane_r; r < r1; r += 16) {
+    float v[kReduceRows / 4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rr = r + 4 * q;
-        if (rr < r1) acc[q] += partial[(size_t)rr * F.P + e];
-      }
-    }
+    for (int q = 0; q < kReduceRows / 4; ++q)
+      v[q] = partial[(size_t)min(r0 + lane_r + 4 * q, r1 - 1) * F.P + e];
+#pragma unroll
+    for (int q = 0; q < kReduceRows / 4; ++q)
+      if (r0 + lane_r + 4 * q < r1) acc[q & 3] += v[q];
   }
   part[lane_r][lane_e] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
@@ -760,6 +1144,9 @@ __global__ __launch_bounds__(256) void reduce_kernel(BwdFrame F, BwdGrads G, int
 struct BwdPlan {
   BwdFrame F;
   size_t lds;
+  bool flat;           // backward_flat_kernel (every node flat, one LDS pass)
+  int rows;            // partial rows the workspace holds at full size
+  size_t row_bytes;    // bytes per partial row
   size_t partial_bytes;
 };
 
@@ -795,13 +1182,76 @@ int plan_backward(const pbr_scene_desc* D, int channels, const pbr_grads* g, Bwd
   F.off_v = kLightF;
   F.off_s = F.off_v + F.vtab * 6;
   F.P = F.off_s + F.stab * 4;
-  P.lds = sizeof(float) * (size_t)F.cap * (kRecF + kAcc) + sizeof(uint32_t) * (size_t)F.cap * 2 +
-          sizeof(uint32_t) * (size_t)F.nwords * 2 + sizeof(float) * 16 +
-          sizeof(float) * (size_t)F.vtab * 6 + sizeof(float) * (size_t)F.stab * 4 +
-          sizeof(uint32_t) * (PBR_MAX_NODES + 1) + sizeof(uint32_t) * (kBwdBlock + 1);
-  P.partial_bytes = ((sizeof(float) * (size_t)F.P * (size_t)D->num_scenes) + 255) / 256 * 256;
+  P.flat = !any_smooth(D) && !F.compact;
+  if (P.flat) {
+    int ntri = 0;
+    for (int m = 0; m < D->num_meshes; ++m)
+      ntri = max(ntri, D->meshes[m].index_offset + D->meshes[m].num_triangles);
+    F.ttab = ntri <= kMaxFlatTtab ? ntri : 0;
+    // acc, light, node table, triangle table, per-node slot counts, lane light
+    // sums, vtab, stab
+    P.lds = sizeof(float) * ((size_t)F.cap * 8 + 16 + PBR_MAX_NODES * 16 + (size_t)F.ttab * 4 +
+                             PBR_MAX_NODES + kLightF * 64 + (size_t)F.vtab * 6 +
+                             (size_t)F.stab * 4);
+  }
+  else
+    P.lds = sizeof(float) * (size_t)F.cap * (kRecF + kAcc) +
+            sizeof(uint32_t) * (size_t)F.cap * 2 + sizeof(uint32_t) * (size_t)F.nwords * 2 +
+            sizeof(float) * 16 + sizeof(float) * (size_t)F.vtab * 6 +
+            sizeof(float) * (size_t)F.stab * 4 + sizeof(uint32_t) * (PBR_MAX_NODES + 1) +
+            sizeof(uint32_t) * (kBwdBlock + 1);
+  // one partial row per workgroup: per scene, or per persistent flat workgroup
+  P.rows = P.flat ? min(D->num_scenes, kMaxFlatGrid) : D->num_scenes;
+  P.row_bytes = sizeof(float) * (size_t)F.P;
+  P.partial_bytes = (P.row_bytes * (size_t)P.rows + 255) / 256 * 256;
   if (P.lds > 160 * 1024)
     return fail(PBR_ERR_UNSUPPORTED, "backward needs %zu bytes of LDS (more than 160 KiB)", P.lds);
+  return PBR_OK;
+}
+
+// Persistent grid of the flat kernel: at most the resident workgroups of the
+// device, kMaxFlatGrid and the partial rows the workspace holds, then evened
+// out so that every workgroup walks the same number of scenes (+-1).
+int flat_grid(const pbr_scene_desc* D, const BwdPlan& P, size_t workspace_bytes,
+              const void* fn) {
+  if (P.lds > 64 * 1024 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds) !=
+          hipSuccess)
+    return 0;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kFlatBlock, P.lds) !=
+          hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  size_t g = (size_t)per_cu * (size_t)device_cus();
+  g = std::min(g, (size_t)kMaxFlatGrid);
+  g = std::min(g, workspace_bytes / P.row_bytes);
+  g = std::min(g, (size_t)D->num_scenes);
+  if (g == 0) return 0;
+  const size_t per = ((size_t)D->num_scenes + g - 1) / g;
+  return (int)(((size_t)D->num_scenes + per - 1) / per);
+}
+
+// Launches backward_flat_kernel<NC>; *grid = its workgroups (0: none fit).
+template <int NC>
+hipError_t launch_flat_t(const pbr_scene_desc* D, const BwdPlan& P, const BwdLight& L,
+                         const BwdGrads& G, const int32_t* vis, const float* gout,
+                         float* partial, size_t workspace_bytes, hipStream_t st, int* grid) {
+  auto k = backward_flat_kernel<NC>;
+  *grid = flat_grid(D, P, workspace_bytes, (const void*)k);
+  if (*grid <= 0) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(k, dim3(*grid), dim3(kFlatBlock), P.lds, st, *D, P.F, L, G, vis, gout,
+                     partial);
+  return hipGetLastError();
+}
+
+int launch_reduce(const pbr_scene_desc* D, const BwdPlan& P, const BwdGrads& G, int rows,
+                  const float* partial, hipStream_t st) {
+  const dim3 rgrid((P.F.P + 63) / 64, (rows + kReduceRows - 1) / kReduceRows);
+  hipLaunchKernelGGL(reduce_kernel, rgrid, dim3(256), 0, st, P.F, G, rows, partial,
+                     D->num_nodes, *D);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "reduce launch failed: %s", hipGetErrorString(e));
   return PBR_OK;
 }
 
@@ -851,7 +1301,10 @@ int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis, const f
   BwdPlan P;
   rc = plan_backward(scene, channels, grads, P);
   if (rc) return rc;
-  if (!workspace || workspace_bytes < P.partial_bytes)
+  // the flat path runs as many workgroups as the workspace holds partial rows
+  // (at least one); the per-scene path needs the full size
+  const size_t need = P.flat ? P.row_bytes : P.partial_bytes;
+  if (!workspace || workspace_bytes < need)
     return fail(PBR_ERR_WORKSPACE, "backward needs a %zu-byte workspace (got %zu)",
                 P.partial_bytes, workspace_bytes);
   if ((uintptr_t)workspace & 255) return fail(PBR_ERR_WORKSPACE, "workspace not 256-byte aligned");
@@ -878,6 +1331,20 @@ int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis, const f
   G.light = grads->light;
   hipStream_t st = (hipStream_t)stream;
   float* partial = reinterpret_cast<float*>(workspace);
+  if (P.flat) {
+    int grid = 0;
+    hipError_t e = hipSuccess;
+    const size_t wb = workspace_bytes;
+    switch (channels) {
+      case 1: e = launch_flat_t<1>(scene, P, L, G, vis, grad_color, partial, wb, st, &grid); break;
+      case 2: e = launch_flat_t<2>(scene, P, L, G, vis, grad_color, partial, wb, st, &grid); break;
+      case 3: e = launch_flat_t<3>(scene, P, L, G, vis, grad_color, partial, wb, st, &grid); break;
+      default: e = launch_flat_t<4>(scene, P, L, G, vis, grad_color, partial, wb, st, &grid); break;
+    }
+    if (e != hipSuccess)
+      return fail(PBR_ERR_LAUNCH, "backward launch failed: %s", hipGetErrorString(e));
+    return launch_reduce(scene, P, G, grid, partial, st);
+  }
   const bool smooth = any_smooth(scene);
   const void* kfn = smooth ? (const void*)backward_kernel<true> : (const void*)backward_kernel<false>;
   if (P.lds > 64 * 1024) {
@@ -894,12 +1361,7 @@ int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis, const f
                        *scene, P.F, L, G, vis, grad_color, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "backward launch failed: %s", hipGetErrorString(e));
-  const dim3 rgrid((P.F.P + 63) / 64, (scene->num_scenes + kReduceRows - 1) / kReduceRows);
-  hipLaunchKernelGGL(reduce_kernel, rgrid, dim3(256), 0, st, P.F, G, scene->num_scenes, partial,
-                     scene->num_nodes, *scene);
-  e = hipGetLastError();
-  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "reduce launch failed: %s", hipGetErrorString(e));
-  return PBR_OK;
+  return launch_reduce(scene, P, G, scene->num_scenes, partial, st);
 }
 
 }  // extern "C"

@@ -77,6 +77,12 @@ def test_backward_validation_without_gpu():
     assert rc == _abi.PBR_ERR_WORKSPACE
     # one partial row of 10 light floats per scene, rounded to 256 bytes
     assert _native.backward_workspace_bytes(d, 3, g) == 256
+    # every node flat: one row per persistent workgroup, at most 2048 of them
+    d.num_scenes = 5000
+    assert _native.backward_workspace_bytes(d, 3, g) == 2048 * 40
+    # ... and any workspace holding one row is accepted; less is refused
+    rc = lib.pbr_render_backward(ctypes.byref(d), 64, 64, 3, ctypes.byref(g), 256, 39, None)
+    assert rc == _abi.PBR_ERR_WORKSPACE and "workspace" in _native.last_error()
 
 
 def test_workspace_size():

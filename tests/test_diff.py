@@ -46,6 +46,31 @@ def mixed_scene(device: str, N: int = 3, res: int = 48, seed: int = 0, channels:
     return r
 
 
+def flat_scene(device: str, N: int = 5, res: int = 48, seed: int = 0, channels: int = 3,
+               box_ips: int = 3) -> PBRRenderer:
+    """Boxes and a shared ground quad: every node shades flat."""
+    g = torch.Generator("cpu").manual_seed(seed)
+    r = PBRRenderer(PBRConfig(num_scenes=N, tile_resolution=(res, res), device=device,
+                              output_dtype="float32", num_channels=channels))
+    box = r.add_node("models/box", instances_per_scene=box_ips)
+    gnd = r.add_node("models/quad", instances_per_scene=1, shared_across_scenes=True)
+    box.set_transforms(random_transforms(N * box_ips, g, pos_range=1.2, scale_range=(1.0, 1.6)))
+    box.set_colors(torch.rand(N * box_ips, 4, generator=g) * 0.8 + 0.1)
+    Mg = torch.eye(4).reshape(1, 4, 4).clone()
+    Mg[0, 0, 0] = Mg[0, 1, 1] = 8.0
+    Mg[0, 2, 3] = -1.5
+    gnd.set_transforms(Mg)
+    gnd.set_colors(torch.tensor([[0.3, 0.6, 0.3, 0.8]]))
+    r.add_camera(fov_y_deg=50.0)
+    eye = torch.tensor([[0.5, -5.0, 2.5]]).expand(N, 3)
+    r._pbr_cam.set_positions_and_lookat(eye, torch.zeros(3))
+    r.add_light(ambient=(0.2, 0.2, 0.25), dir_dir=(0.4, -0.6, 0.7), dir_col=(0.9, 0.8, 0.7),
+                strength=0.9)
+    r.setup_environment()
+    assert all(r._geometry["flat"])
+    return r
+
+
 def _node_masks(ds: DenseScene, vis: torch.Tensor) -> list[torch.Tensor]:
     base = ds._slot_base()
     v = vis.long()
@@ -235,6 +260,83 @@ def test_backward_cartpole_ground_scene(cuda):
     r._step(st.cuda())
     got, want, _ = _kernel_and_reference(r, seed=8)
     _assert_close(got, want, label="cartpole")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("channels", [1, 2, 3, 4])
+def test_backward_flat_scene(cuda, channels):
+    """Every node flat: the persistent flat kernel."""
+    r = flat_scene("cuda", channels=channels, seed=channels)
+    got, want, _ = _kernel_and_reference(r, seed=10 + channels)
+    _assert_close(got, want, label=f"flat C={channels}")
+    assert got["vertices"].abs().sum() > 0 and got["color0"].abs().sum() > 0
+
+
+@pytest.mark.gpu
+def test_backward_flat_uneven_counts(cuda):
+    r = flat_scene("cuda", N=6, seed=11, box_ips=4)
+    r._pbr_nodes[0].set_instance_counts(torch.tensor([0, 4, 1, 3, 2, 4], dtype=torch.int32))
+    r._mark_geometry_dirty()
+    r.setup_environment()
+    got, want, _ = _kernel_and_reference(r, seed=12)
+    _assert_close(got, want, label="flat uneven")
+
+
+def _abi_backward(r: PBRRenderer, gout: torch.Tensor, rows: int | None = None) -> dict:
+    """pbr_render_backward through the C ABI with a workspace of `rows`
+    partial rows (None: the size pbr_backward_workspace_bytes asks for)."""
+    from pybatchrender_amd import _abi, _native
+    p = diff.scene_params(r, requires_grad=False)
+    desc, keep = diff._descriptor(r, p.vertices, p.light, p.colors)
+    vis = r.render(vis=True)["vis"].contiguous()
+    C = gout.shape[1]
+    gcols = [torch.zeros_like(c) for c in p.colors]
+    gverts = torch.zeros_like(p.vertices)
+    glight = torch.zeros(10, device="cuda")
+    g = _abi.pbr_grads()
+    for j, t in enumerate(gcols):
+        g.colors[j] = t.data_ptr()
+    g.vertices, g.light = gverts.data_ptr(), glight.data_ptr()
+    full = _native.backward_workspace_bytes(desc, C, g)
+    nbytes = full
+    if rows is not None:
+        shared = sum(nd.instances_per_scene for nd in desc.nodes[:desc.num_nodes] if nd.shared)
+        nbytes = rows * 4 * (10 + 6 * p.vertices.shape[0] + 4 * shared)
+        assert nbytes <= full
+    ws = torch.empty(max(full, 1), dtype=torch.uint8, device="cuda")
+    go = gout.to("cuda").contiguous()
+    stream = torch.cuda.current_stream().cuda_stream
+    _native.render_backward(desc, vis.data_ptr(), go.data_ptr(), C, g, ws.data_ptr(), nbytes,
+                            stream)
+    torch.cuda.synchronize()
+    del keep
+    return {"light": glight.cpu(), "vertices": gverts.cpu(),
+            **{f"color{j}": t.cpu() for j, t in enumerate(gcols)}}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows", [1, 3])
+def test_backward_flat_small_workspace(cuda, rows):
+    """A workspace of `rows` partial rows runs that many workgroups, each
+    walking several scenes through both halves of its accumulator table."""
+    r = flat_scene("cuda", N=11, res=40, seed=13)
+    N, C, H, W = r.output_shape()
+    gout = torch.randn(N, C, H, W, generator=torch.Generator().manual_seed(14))
+    want = _abi_backward(r, gout)
+    got = _abi_backward(r, gout, rows=rows)
+    _assert_close(got, {k: v.double() for k, v in want.items()}, rel=1e-4,
+                  label=f"{rows} rows")
+    # and the full-size call against the float64 reference
+    ds = DenseScene(*r.scene_descriptor())
+    vis = r.render(vis=True)["vis"].cpu()
+    p = diff.scene_params(r, requires_grad=False)
+    cols = [c.detach().cpu().double().requires_grad_(True) for c in p.colors]
+    verts = p.vertices.detach().cpu().double().requires_grad_(True)
+    lt = p.light.detach().double().requires_grad_(True)
+    loss_fn(ds, vis, gout.double(), cols, verts, lt, C).backward()
+    ref = {"light": lt.grad, "vertices": verts.grad,
+           **{f"color{j}": c.grad for j, c in enumerate(cols)}}
+    _assert_close(got, ref, label=f"{rows} rows vs reference")
 
 
 @pytest.mark.gpu
