@@ -826,7 +826,12 @@ __device__ __forceinline__ void flat_runs(const BwdFrame& F, const int32_t* vsce
 #pragma unroll
       for (int c = 0; c < NC; ++c) a[c] += gg[c][i];
     }
-    // end of run: two wave-wide rounds for the most common open slots
+    // end of run: two wave-wide rounds for the most common open slots.  The
+    // single lane's add goes through a lane-varying (zero) offset: with a
+    // uniform address the compiler would wrap it in a wave reduction of its
+    // own.
+    int zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
 #pragma unroll
     for (int round = 0; round < 2; ++round) {
       const uint64_t open = __ballot(cur >= 0);
@@ -836,7 +841,7 @@ __device__ __forceinline__ void flat_runs(const BwdFrame& F, const int32_t* vsce
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         const float t = wave_sum63(mine ? a[c] : 0.0f);
-        if (lane == 63) atomicAdd(acc + (size_t)lead * 4 + c, t);
+        if (lane == 63) atomicAdd(acc + (size_t)(lead * 4 + c + zero), t);
       }
       if (mine) cur = -1;
     }
