@@ -417,6 +417,8 @@ def main() -> int:
     if args.bwd_only:
         log("[bench] fwd+bwd leg only")
         print(json.dumps(fwd_bwd_leg(n, states[0])), flush=True)
+        if args.copy_calibration:
+            copy_calibration()
         return 0
     log(f"[bench] eager: warmup {args.warmup} steps, N={n}")
     for i in range(args.warmup):
