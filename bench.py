@@ -19,7 +19,8 @@ Printed JSON line (one, from this single process):
                around back-to-back calls on the launch stream, vs 8 TB/s
   cpu_baseline the CPU oracle (a C restatement of the same render path)
                rendering a 256-scene sample of the same workload on all host
-               cores
+               cores: median render (p10/p90 beside it) and a one-thread
+               median
   fwd_bwd      the same workload with float32 output: plain forward vs
                forward with the visibility buffer + pbr_render_backward
                (colour, light and vertex gradients), C-ABI launches timed
@@ -320,13 +321,33 @@ def copy_calibration(mib: int = 256, reps: int = 5) -> None:
     print(json.dumps({"copy_bytes": mib << 20, "reps": reps}), flush=True)
 
 
-def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 10.0) -> dict:
-    """The CPU oracle on a bounded sample: the first `sample_scenes` scenes."""
+def _oracle_render_times(lib, desc, fo, threads: int, min_seconds: float,
+                         min_iters: int = 20) -> list[float]:
+    """Seconds per oracle render of `desc` on `threads` threads: 3 warm-up
+    renders, then at least `min_iters` timed ones and `min_seconds` of work."""
+    for _ in range(3):
+        lib.pbr_oracle_render_mt(ctypes.byref(desc), ctypes.byref(fo), threads)
+    times, start = [], time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        lib.pbr_oracle_render_mt(ctypes.byref(desc), ctypes.byref(fo), threads)
+        t1 = time.perf_counter()
+        times.append(t1 - t0)
+        if t1 - start >= min_seconds and len(times) >= min_iters:
+            return times
+
+
+def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 8.0,
+                 single_seconds: float = 3.0) -> dict:
+    """The CPU oracle on a bounded sample: the first `sample_scenes` scenes,
+    timed per render on every host core the process may use (the reported
+    value, from the median render) and on one thread."""
     from oracle import oracle
 
     desc, keep = r.scene_descriptor()
     hs = oracle.HostScene(desc, keep)
     hs.desc.num_scenes = sample_scenes
+    import numpy as np
     import torch
 
     from pybatchrender_amd import _abi
@@ -337,17 +358,16 @@ def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 10.0) -> dict
     fo.color, fo.channels, fo.dtype = pix.data_ptr(), C, _abi.PBR_DTYPE_U8
     threads = len(os.sched_getaffinity(0))
     lib = oracle.library()
-    lib.pbr_oracle_render_mt(ctypes.byref(hs.desc), ctypes.byref(fo), threads)  # warmup
-    iters, t0 = 0, time.perf_counter()
-    while True:
-        lib.pbr_oracle_render_mt(ctypes.byref(hs.desc), ctypes.byref(fo), threads)
-        iters += 1
-        el = time.perf_counter() - t0
-        if el >= min_seconds and iters >= 3:
-            break
-    return {"value": N * iters / el, "unit": "scene-frames/s", "cores": threads, "kind": "port",
-            "sample": f"{N} scenes of the same workload (first {N} states), {iters} renders, "
-                      f"{el:.1f} s, C oracle on {threads} threads"}
+    t_mt = np.array(_oracle_render_times(lib, hs.desc, fo, threads, min_seconds))
+    t_1 = np.array(_oracle_render_times(lib, hs.desc, fo, 1, single_seconds, min_iters=5))
+    p10, p50, p90 = np.percentile(t_mt, [10, 50, 90])
+    return {"value": float(N / p50), "unit": "scene-frames/s", "cores": threads, "kind": "port",
+            "p10_p90": [round(float(N / p90), 1), round(float(N / p10), 1)],
+            "single_thread": round(N / float(np.median(t_1)), 1),
+            "sample": f"{N} scenes of the same workload (first {N} states); median of "
+                      f"{len(t_mt)} renders on {threads} threads ({t_mt.sum():.1f} s) after 3 "
+                      f"warm-up renders; single_thread: median of {len(t_1)} renders on 1 "
+                      f"thread; C oracle"}
 
 
 def env_step_rate(n: int, steps: int) -> float:
