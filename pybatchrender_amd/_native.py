@@ -28,6 +28,18 @@ def library() -> ctypes.CDLL:
     return _LIB
 
 
+def stream_ptr(device) -> int:
+    """Raw HIP stream handle of torch's current stream on `device` (a
+    torch.device of type cuda)."""
+    import torch
+
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return raw(idx)
+    return torch.cuda.current_stream(device).cuda_stream
+
+
 def last_error() -> str:
     msg = library().pbr_last_error()
     return msg.decode() if msg else ""

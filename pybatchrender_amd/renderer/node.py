@@ -57,6 +57,7 @@ class PBRNode(PBRShaderContext):
             raise NotImplementedError("PBRNode parent is not implemented (node.py:29)")
         super().__init__(showbase, backend=backend)
         self.device = torch.device(getattr(showbase, "device", "cpu"))
+        self._pose_cache = None  # GPU pose descriptor of the output buffers (_pose_inputs_gpu)
         self.num_scenes = int(num_scenes)
         self.instances_per_scene = int(instances_per_scene)
         if self.instances_per_scene < 0:
@@ -184,6 +185,8 @@ class PBRNode(PBRShaderContext):
         the full per-row component tensors (CPU)."""
         if not self.has_geometry:
             return None
+        if self.device.type == "cuda":
+            return self._pose_inputs_gpu(comps)
         B = self.buf_instances
         dev = self.device
         if self.transforms_b44.shape != (B, 4, 4) or not self.transforms_b44.is_contiguous():
@@ -222,6 +225,53 @@ class PBRNode(PBRShaderContext):
                 keep.append(v)
             else:
                 c.src, c.stride, c.value = None, 0, v
+        return d, keep
+
+    def _pose_inputs_gpu(self, comps: dict):
+        """_pose_inputs on a GPU device: the pose descriptor and the tensors
+        it points to.  The output-buffer part of the descriptor is kept per
+        node and reused while the buffers are the same objects (a renderer
+        step poses every dynamic node, so this runs per step)."""
+        from .. import _abi
+
+        B = self.buf_instances
+        pc = self._pose_cache
+        if (pc is None or pc[0] is not self.transforms_b44 or pc[1] is not self.rot3_b33
+                or pc[2] is not self.scale_b11 or pc[3] != B):
+            dev = self.device
+            if self.transforms_b44.shape != (B, 4, 4) or not self.transforms_b44.is_contiguous():
+                self.transforms_b44 = torch.eye(4, dtype=torch.float32, device=dev).repeat(B, 1, 1)
+            if self.rot3_b33.shape != (B, 3, 3) or not self.rot3_b33.is_contiguous():
+                self.rot3_b33 = torch.empty((B, 3, 3), dtype=torch.float32, device=dev)
+            if self.scale_b11.shape != (B, 1, 1) or not self.scale_b11.is_contiguous():
+                self.scale_b11 = torch.empty((B, 1, 1), dtype=torch.float32, device=dev)
+            t = _abi.pbr_pose_desc()
+            t.transforms = self.transforms_b44.data_ptr()
+            t.rot3 = self.rot3_b33.data_ptr()
+            t.scale = self.scale_b11.data_ptr()
+            t.rows = B
+            pc = self._pose_cache = (self.transforms_b44, self.rot3_b33, self.scale_b11, B, t)
+        d = _abi.pbr_pose_desc.from_buffer_copy(pc[4])
+        dev_index = self.device.index
+        keep = []
+        for k, v in comps.items():
+            c = getattr(d, k)
+            if isinstance(v, torch.Tensor):
+                if v.dtype is not torch.float32 or not v.is_cuda or v.get_device() != dev_index:
+                    v = v.to(self.device, torch.float32)
+                if v.dim() != 1:
+                    v = v.reshape(-1)
+                n = v.numel()
+                if n < B:
+                    if n != 1:
+                        raise ValueError(f"pose component {k}: {n} values for {B} rows")
+                    v = v.expand(B)
+                elif n > B:
+                    v = v[:B]
+                c.src, c.stride, c.value = v.data_ptr(), v.stride(0), 0.0
+                keep.append(v)
+            else:
+                c.src, c.stride, c.value = None, 0, float(v)
         return d, keep
 
     def _pose_host(self, full: dict) -> None:

@@ -56,6 +56,11 @@ class PBRRenderer:
         self._geometry = None
         self._workspace: torch.Tensor | None = None
         self._env_ready = False
+        # scene descriptor of the last call: (key, desc, keep), and the render
+        # options / workspace size derived from it (per-step host cost)
+        self._desc_cache = None
+        self._opts_cache = None
+        self._ws_cache = None
 
     # --- scene construction --------------------------------------------------
     def set_background_color(self, r: float, g: float, b: float, a: float = 1.0) -> None:
@@ -148,12 +153,48 @@ class PBRRenderer:
         }
 
     # --- the C-ABI descriptor ------------------------------------------------------
+    def _descriptor_key(self) -> list:
+        """Everything scene_descriptor reads, as objects compared by identity
+        (tensors: the descriptor holds their pointers, so in-place updates
+        keep it valid) or by value (numbers, tuples)."""
+        g = self._geometry
+        L = self._pbr_light
+        key = [g, self.num_scenes, self.cfg.tile_resolution, self._pbr_cam.VP_k44, L,
+               self._background_color]
+        if L is not None:
+            key += [L.dir_dir, L.dir_col, L.ambient, L.strength]
+        for n in g["nodes"]:
+            key += [n.transforms_b44, n.colors_b4, n.instance_counts, n.instances_per_scene,
+                    n.shared_across]
+        return key
+
+    @staticmethod
+    def _same_key(a: list, b: list) -> bool:
+        if len(a) != len(b):
+            return False
+        for x, y in zip(a, b):
+            if x is y:
+                continue
+            if isinstance(x, torch.Tensor) or isinstance(y, torch.Tensor) or x != y:
+                return False
+        return True
+
     def scene_descriptor(self) -> tuple[_abi.pbr_scene_desc, list[torch.Tensor]]:
-        """pbr_scene_desc of the current state plus the tensors it points to."""
+        """pbr_scene_desc of the current state plus the tensors it points to
+        (a fresh copy per call; rebuilt only when a buffer, the camera's
+        view-projection tensor, the light or the background changed)."""
         if self._geometry is None:
             self._build_geometry()
         if self._pbr_cam is None:
             self.add_camera()
+        key = self._descriptor_key()
+        c = self._desc_cache
+        if c is None or not self._same_key(c[0], key):
+            d, keep = self._build_scene_descriptor()
+            c = self._desc_cache = (key, d, keep, self.slot_budget())
+        return _abi.pbr_scene_desc.from_buffer_copy(c[1]), list(c[2])
+
+    def _build_scene_descriptor(self) -> tuple[_abi.pbr_scene_desc, list[torch.Tensor]]:
         g = self._geometry
         N = self.num_scenes
         H, W = int(self.cfg.tile_resolution[1]), int(self.cfg.tile_resolution[0])
@@ -244,15 +285,19 @@ class PBRRenderer:
         fo.vis = vis.data_ptr() if isinstance(vis, torch.Tensor) else None
         fo.channels = C
         fo.dtype = code
-        opts = self.render_opts()
-        ws_bytes = _native.workspace_bytes(desc, opts)
+        opts = self._cached_opts()
+        wc = self._ws_cache
+        if wc is not None and wc[0] is self._desc_cache and wc[1] is opts:
+            ws_bytes = wc[2]
+        else:
+            ws_bytes = _native.workspace_bytes(desc, opts)
+            self._ws_cache = (self._desc_cache, opts, ws_bytes)
         ws_ptr = 0
         if ws_bytes:
             if self._workspace is None or self._workspace.numel() < ws_bytes:
                 self._workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
             ws_ptr = self._workspace.data_ptr()
-        stream = torch.cuda.current_stream(self.device).cuda_stream
-        _native.render(desc, fo, opts, ws_ptr, ws_bytes, stream)
+        _native.render(desc, fo, opts, ws_ptr, ws_bytes, _native.stream_ptr(self.device))
         del keep
         result = {"pixels": out}
         if isinstance(depth, torch.Tensor):
@@ -260,6 +305,22 @@ class PBRRenderer:
         if isinstance(vis, torch.Tensor):
             result["vis"] = vis
         return result
+
+    def _opts_key(self) -> tuple:
+        c = self.cfg
+        return (c.region_rows, c.persistent, getattr(self, "workgroups", 0), c.kernel, c.bin_width,
+                c.bins_per_item, c.lane_pixels, c.tile_size, c.row_list_cap, self._desc_cache[3])
+
+    def _cached_opts(self) -> _abi.pbr_render_opts:
+        """render_opts() for the current configuration, rebuilt when a field
+        it reads changed (the returned object is shared: do not modify).
+        Call after scene_descriptor(), whose cache entry holds the slot
+        budget of the current instance counts."""
+        key = self._opts_key()
+        c = self._opts_cache
+        if c is None or c[0] != key:
+            c = self._opts_cache = (key, self.render_opts())
+        return c[1]
 
     def render_opts(self) -> _abi.pbr_render_opts:
         o = _abi.pbr_render_opts()
@@ -298,10 +359,9 @@ class PBRRenderer:
                     for node, kw in poses]
         prepared = [(n, p) for n, p in prepared if p is not None]
         if self.device.type == "cuda":
+            stream = _native.stream_ptr(self.device)
             for k in range(0, len(prepared), 8):
-                batch = prepared[k:k + 8]
-                _native.set_poses([p[0] for _, p in batch],
-                                  torch.cuda.current_stream(self.device).cuda_stream)
+                _native.set_poses([p[0] for _, p in prepared[k:k + 8]], stream)
             return
         for node, full in prepared:
             node._pose_host(full)
@@ -349,6 +409,7 @@ class PBRRenderer:
     def destroy(self) -> None:
         self._geometry = None
         self._workspace = None
+        self._desc_cache = self._opts_cache = self._ws_cache = None
 
 
 class GraphedStep:

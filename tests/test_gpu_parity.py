@@ -447,6 +447,18 @@ def test_pose_kernel_matches_host_formula(cuda):
     ng.set_pose(x=1.5, p=stg[:, 4])  # constants + one column
     nc.set_pose(x=1.5, p=st[:, 4])
     assert torch.allclose(ng.transforms_b44.cpu(), nc.transforms_b44, atol=2e-6)
+    # a one-element tensor (broadcast), a longer column (first rows), float64
+    # and host tensors (converted), then the output buffers replaced
+    long = torch.rand(n + 17, generator=g) - 0.5
+    ng.set_pose(x=torch.tensor([0.75], device="cuda"), y=long.cuda(), h=st[:, 3].double().cuda(),
+                r=st[:, 5])
+    nc.set_pose(x=0.75, y=long[:n], h=st[:, 3], r=st[:, 5])
+    assert torch.allclose(ng.transforms_b44.cpu(), nc.transforms_b44, atol=2e-6)
+    ng.transforms_b44 = torch.zeros(1, device="cuda")
+    ng.set_pose(z=stg[:, 2])
+    nc.set_pose(z=st[:, 2])
+    assert ng.transforms_b44.shape == (n, 4, 4)
+    assert torch.allclose(ng.transforms_b44.cpu(), nc.transforms_b44, atol=2e-6)
 
 
 @pytest.mark.parametrize("scene", ["ground", "reference"])

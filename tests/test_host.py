@@ -153,6 +153,43 @@ def test_shared_node_rows_and_counts_in_descriptor():
     assert tuple(round(v, 4) for v in d.background) == (0.41, 0.41, 0.41, 1.0)
 
 
+def test_descriptor_cache_follows_every_input():
+    """scene_descriptor reuses its last build only while every buffer it
+    points to is the same tensor and the light / background are unchanged;
+    each call returns its own copy."""
+    r = PBRRenderer(PBRConfig(num_scenes=3, device="cpu"))
+    box = r.add_node("models/box", instances_per_scene=2)
+    r.add_light(strength=0.5)
+    r.setup_environment()
+    d1, k1 = r.scene_descriptor()
+    d1.strength = 7.0  # the caller's copy only
+    k1.append(torch.zeros(1))
+    d2, k2 = r.scene_descriptor()
+    assert d2.strength == 0.5 and len(k2) == len(k1) - 1
+    assert bytes(d2) == bytes(r._build_scene_descriptor()[0])
+    # light, background, camera (new VP tensor), node buffers, counts, geometry
+    r._pbr_light.set_strength(0.25)
+    assert r.scene_descriptor()[0].strength == 0.25
+    r.set_background_color(0.1, 0.2, 0.3)
+    assert round(r.scene_descriptor()[0].background[2], 5) == 0.3
+    r._pbr_cam.set_positions(torch.tensor([1.0, -9.0, 2.0]))
+    assert r.scene_descriptor()[0].view_proj == r._pbr_cam.VP_k44.data_ptr()
+    box.set_colors(torch.rand(6, 4))
+    assert r.scene_descriptor()[0].nodes[0].colors == box.colors_b4.data_ptr()
+    box.set_instance_counts(torch.tensor([1, 2, 0]))
+    d, _ = r.scene_descriptor()
+    assert d.nodes[0].counts == box.instance_counts.data_ptr()
+    assert r._desc_cache[3] == r.slot_budget() == 3 * 12
+    r.add_node("models/quad", instances_per_scene=1, shared_across_scenes=True)
+    r.setup_environment()
+    assert r.scene_descriptor()[0].num_nodes == 2
+    # in-place updates keep the descriptor (it holds the pointers)
+    before = r._desc_cache
+    box.colors_b4.mul_(0.5)
+    r.scene_descriptor()
+    assert r._desc_cache is before
+
+
 def test_too_few_rows_rejected():
     r = PBRRenderer(PBRConfig(num_scenes=4, device="cpu"))
     node = r.add_node("models/box", instances_per_scene=1)
