@@ -63,14 +63,46 @@ def seeded_states(n: int, batches: int, seed: int = 0):
 
 
 def algorithmic_bytes(r) -> int:
-    """Pixels written + per-scene instance rows read (80 B each) + VP (64 B)."""
+    """Pixels (and depth) written + the instance rows drawn (80 B each) + VP
+    (64 B) per scene; shared rows, meshes and the light are read once."""
     N, C, H, W = r.output_shape()
     esize = {"uint8": 1, "bfloat16": 2, "float32": 4}[r.cfg.output_dtype]
-    inst = sum(n.instances_per_scene for n in r._drawn_nodes() if not n.shared_across)
-    per_scene = H * W * C * esize + 80 * inst + 64
+    per_scene = H * W * C * esize + 64
     if r.cfg.output_depth:
         per_scene += H * W * 4
-    return N * per_scene
+    inst = 0
+    for n in r._drawn_nodes():
+        if n.shared_across:
+            continue
+        if n.instance_counts is not None:
+            inst += int(n.instance_counts.sum().item())
+        else:
+            inst += N * n.instances_per_scene
+    return N * per_scene + 80 * inst
+
+
+def triangles_drawn(r) -> int:
+    t = 0
+    for n in r._drawn_nodes():
+        if n.instance_counts is not None:
+            cnt = int(n.instance_counts.sum().item())
+        else:
+            cnt = n.instances_per_scene * (1 if n.shared_across else r.num_scenes)
+        t += cnt * n.mesh.num_triangles
+    return t
+
+
+def config_renderer(name: str, persistent: bool):
+    """The renderer of a named workload for --kernel-only timing."""
+    from pybatchrender_amd import workloads
+
+    if name == "many-cubes":
+        return workloads.many_cubes(persistent=persistent)
+    if name == "uneven":
+        return workloads.uneven_batch(persistent=persistent)
+    if name == "uneven-uniform":
+        return workloads.uneven_batch(persistent=persistent, uniform=True)
+    raise ValueError(name)
 
 
 def kernel_time_us(r, launches: int = 200) -> float:
@@ -410,6 +442,15 @@ def main() -> int:
                     help="only the fwd+bwd leg (for kernel profiles of the backward)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="with --kernel-only: also copy 256 MiB device-to-device 5 times")
+    ap.add_argument("--config", default="cartpole",
+                    choices=["cartpole", "many-cubes", "uneven", "uneven-uniform"],
+                    help="with --kernel-only: the workload.  cartpole: 4096 CartPole scenes "
+                         "64x64 uint8; many-cubes: 1024 scenes 128x128 float32 RGB + depth, 64 "
+                         "random boxes each; uneven: 8192 scenes 84x84 uint8, 1-256 instances "
+                         "(log-uniform) of box / sphere / cylinder; uneven-uniform: the same "
+                         "draws with every scene at the mean instance count")
+    ap.add_argument("--persistent", action="store_true",
+                    help="with --config other than cartpole: persistent work queue")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this renderer shards scenes with no "
@@ -423,6 +464,20 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
+    if args.kernel_only and args.config != "cartpole":
+        log(f"[bench] kernel timing only: {args.config}")
+        rc_ = config_renderer(args.config, args.persistent)
+        kus = kernel_time_us(rc_, launches=args.steps)
+        tri = triangles_drawn(rc_)
+        nb = algorithmic_bytes(rc_)
+        print(json.dumps({"config": args.config, "persistent": bool(args.persistent),
+                          "kernel": render_kernel_name(rc_), "kernel_us": round(kus, 2),
+                          "scene_frames_per_s": round(rc_.num_scenes / kus * 1e6, 1),
+                          "triangles": tri, "ns_per_triangle": round(kus * 1e3 / tri, 4),
+                          "bytes_per_launch": nb, "gbs": round(nb / kus / 1e3, 1)}), flush=True)
+        if args.copy_calibration:
+            copy_calibration()
+        return 0
     n = args.scenes
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda", kernel=args.kernel))
     states = seeded_states(n, args.warmup + args.steps).cuda()
