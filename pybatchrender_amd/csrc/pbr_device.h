@@ -523,25 +523,64 @@ __device__ __forceinline__ float pose_value(const pbr_pose_component& c, int64_t
   return c.src ? c.src[i * c.stride] : c.value;
 }
 
-__device__ __forceinline__ void write_pose(const pbr_pose_desc& P, int64_t i) {
+// cos and sin of angle component c at row i; a constant +0 angle skips the
+// evaluation (cosf(+0) = 1 and sinf(+0) = +0 exactly).
+__device__ __forceinline__ void pose_cos_sin(const pbr_pose_component& c, int64_t i, float& cs,
+                                             float& sn) {
+  if (!c.src && __float_as_uint(c.value) == 0u) {
+    cs = 1.0f;
+    sn = 0.0f;
+    return;
+  }
+  const float a = pose_value(c, i);
+  cs = cosf(a);
+  sn = sinf(a);
+}
+
+// Row i of P: rows 0-2 of M, R and s.
+__device__ __forceinline__ void pose_row(const pbr_pose_desc& P, int64_t i, float4 M[3],
+                                         float R[9], float& sc) {
   const float x = pose_value(P.x, i), y = pose_value(P.y, i), z = pose_value(P.z, i);
-  const float h = pose_value(P.h, i), p = pose_value(P.p, i), r = pose_value(P.r, i);
-  const float sc = pose_value(P.s, i);
-  const float ch = cosf(h), sh = sinf(h), cp = cosf(p), sp = sinf(p), cr = cosf(r), sr = sinf(r);
+  sc = pose_value(P.s, i);
+  float ch, sh, cp, sp, cr, sr;
+  pose_cos_sin(P.h, i, ch, sh);
+  pose_cos_sin(P.p, i, cp, sp);
+  pose_cos_sin(P.r, i, cr, sr);
   const float chsp = ch * sp, shsp = sh * sp;
-  const float R[9] = {ch * cp, chsp * sr - sh * cr, chsp * cr + sh * sr,
-                      sh * cp, shsp * sr + ch * cr, shsp * cr - ch * sr,
-                      -sp,     cp * sr,             cp * cr};
-  float4* M = reinterpret_cast<float4*>(P.transforms + i * 16);
+  R[0] = ch * cp;
+  R[1] = chsp * sr - sh * cr;
+  R[2] = chsp * cr + sh * sr;
+  R[3] = sh * cp;
+  R[4] = shsp * sr + ch * cr;
+  R[5] = shsp * cr - ch * sr;
+  R[6] = -sp;
+  R[7] = cp * sr;
+  R[8] = cp * cr;
   M[0] = make_float4(R[0] * sc, R[1] * sc, R[2] * sc, x);
   M[1] = make_float4(R[3] * sc, R[4] * sc, R[5] * sc, y);
   M[2] = make_float4(R[6] * sc, R[7] * sc, R[8] * sc, z);
-  M[3] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+}
+
+// Stores row i of P's outputs (transforms, and rot3 / scale when present).
+__device__ __forceinline__ void store_pose(const pbr_pose_desc& P, int64_t i, const float4 M[3],
+                                           const float R[9], float sc) {
+  float4* Mo = reinterpret_cast<float4*>(P.transforms + i * 16);
+  Mo[0] = M[0];
+  Mo[1] = M[1];
+  Mo[2] = M[2];
+  Mo[3] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (P.rot3) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) P.rot3[i * 9 + k] = R[k];
   }
   if (P.scale) P.scale[i] = sc;
+}
+
+__device__ __forceinline__ void write_pose(const pbr_pose_desc& P, int64_t i) {
+  float4 M[3];
+  float R[9], sc;
+  pose_row(P, i, M, R, sc);
+  store_pose(P, i, M, R, sc);
 }
 
 }  // namespace
