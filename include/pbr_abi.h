@@ -246,7 +246,9 @@ int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts
  * basic.frag:31-38 computes them (per-face normal for flat nodes,
  * perspective-correct interpolated normals for smooth ones, weights from the
  * unsnapped window positions).  `workspace` holds at least
- * pbr_backward_workspace_bytes() bytes.
+ * pbr_backward_workspace_bytes() bytes; when every node is flat and a scene
+ * has at most 512 triangle slots, any workspace holding one partial row is
+ * accepted (fewer rows run fewer persistent workgroups).
  * The reference has no backward (Panda3D/GL is not differentiable); this
  * serves callers that fit scene parameters to images. */
 int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis,
