@@ -40,7 +40,11 @@ namespace {
 constexpr int kTileBlock = 512;
 constexpr int kPerThread = 2;                        // boxes one thread tests per round
 constexpr int kRound = kTileBlock * kPerThread;      // boxes per round
-constexpr int kQCap = 2 * kRound;                    // entries per LDS queue
+// entries per LDS queue: a round adds at most kRound, so a queue holding more
+// than kQCap - kRound entries is flushed before the next round.  1904: 64 KiB
+// of pixel keys, both queues, the record staging slots and the static LDS
+// then stay under half of the CU's 160 KiB, so two workgroups share a CU.
+constexpr int kQCap = 2 * kRound - 144;
 constexpr int kAutoTilePixels = 8192;                // auto tiles: 64 KiB of pixel state
 constexpr int kSmallArea = 48;   // box-in-tile pixels one lane rasterises alone
 constexpr uint32_t kDepthOne = 0x3f800000u;
@@ -317,8 +321,20 @@ __device__ __forceinline__ void raster_wave(unsigned long long* pix, const TileC
   if (a0 > a1 || b0 > b1) return;
   const int w = a1 - a0 + 1;
   const int npx = w * (b1 - b0 + 1);
+  // p -> (p % w, p / w) from a float reciprocal: p < 2^24 and w < 2^16, so
+  // the truncated quotient is off by at most one and one step corrects it
+  const float inv_w = 1.0f / (float)w;
   for (int p = lane; p < npx; p += 64) {
-    const int i = a0 + p % w, j = b0 + p / w;
+    int q = (int)((float)p * inv_w);
+    int r = p - q * w;
+    if (r < 0) {
+      --q;
+      r += w;
+    } else if (r >= w) {
+      ++q;
+      r -= w;
+    }
+    const int i = a0 + r, j = b0 + q;
     const double px = (double)(256 * i + 128), py = (double)(256 * j + 128);
     const double e0 = fma(A[0], px, fma(B[0], py, Cc[0]));
     const double e1 = fma(A[1], px, fma(B[1], py, Cc[1]));
@@ -376,6 +392,22 @@ __device__ __noinline__ void raster_clipped(unsigned long long* pix, const TileC
   }
 }
 
+// Starts copying record g into the 64-byte LDS slot `dst` without passing
+// through registers: lanes 0-3 each move 16 bytes (global_load_lds_dwordx4
+// writes lane l's data at dst + 16 l).  Complete after vmcnt reaches 0.
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void global_void;
+__device__ __forceinline__ void stage_grec(const GRec* g, uint4* dst, int lane) {
+  if (lane < 4)
+    __builtin_amdgcn_global_load_lds((global_void*)(reinterpret_cast<const uint4*>(g) + lane),
+                                     (lds_void*)dst, 16, 0, 0);
+}
+
+__device__ __forceinline__ void wait_staged() {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the wave's staged copies have landed
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 __device__ __forceinline__ void load_grec(const GRec* g, GRec& r) {
   const uint4* src = reinterpret_cast<const uint4*>(g);
   uint4* dst = reinterpret_cast<uint4*>(&r);
@@ -395,6 +427,7 @@ __global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc 
   uint32_t* qb = qs + kQCap;
   __shared__ uint32_t ns, nb, item_sh;
   __shared__ uint32_t base[PBR_MAX_NODES + 1];
+  __shared__ __align__(16) uint4 stage[kTileBlock / 64][2][4];  // per wave: two records
   const int tid = (int)threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int W = F.W, H = F.H;
@@ -476,10 +509,24 @@ __global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc 
         }
       }
       if (flush_b) {
-        for (uint32_t k = (uint32_t)wave; k < cb; k += kTileBlock / 64) {
+        // each wave walks its share of the large queue; the next record is
+        // copied into the wave's other LDS slot while this one is rasterised
+        const int wv = __builtin_amdgcn_readfirstlane(wave);
+        uint32_t k = (uint32_t)wv;
+        int buf = 0;
+        if (k < cb) stage_grec(P.recs + t.off + qb[k], stage[wv][0], lane);
+        for (; k < cb; k += kTileBlock / 64) {
           const uint32_t sl = qb[k];
+          wait_staged();
           GRec R;
-          load_grec(P.recs + t.off + sl, R);
+          {
+            uint4* dst = reinterpret_cast<uint4*>(&R);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dst[q] = stage[wv][buf][q];
+          }
+          const uint32_t kn = k + kTileBlock / 64;
+          if (kn < cb) stage_grec(P.recs + t.off + qb[kn], stage[wv][buf ^ 1], lane);
+          buf ^= 1;
           if (R.flags & kFlagClipped)
             raster_clipped(pix, t, D, F, base, sl, lane);
           else
