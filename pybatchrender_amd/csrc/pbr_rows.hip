@@ -347,6 +347,40 @@ __device__ __forceinline__ void edges_of(const FixTri& T, RowRec& R) {
   R.zc = T.zc;
 }
 
+// The launch's scene descriptor where the kernel arguments lie (it is the
+// first argument of rows_kernel, so it starts the kernarg segment), as a
+// plain pointer for out-of-line code: taking the address of the by-value
+// argument would make each lane copy it into private memory first.
+__device__ __forceinline__ const pbr_scene_desc* kernarg_scene() {
+  return reinterpret_cast<const pbr_scene_desc*>(
+      (const void*)__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+// Row-list entries at GL row j of the clipped triangle (node, local) of
+// scene s: their number, and with `ent` set the entries themselves
+// (lo | hi << 8 | (rref | sub-triangle) << 16, tz, zx, colw), at most
+// `room` of them.  Out of line, with the descriptor passed by address: only
+// triangles that straddle a clip plane get here, and the vertex fetch and
+// clipper then keep their registers out of the row-span kernel's own.
+__device__ __noinline__ int clipped_row_entries(const pbr_scene_desc* D, RowsFrame F, int s,
+                                                int node, uint32_t local, int j, uint4* ent,
+                                                int room, uint32_t rref, uint32_t colw) {
+  CVert cv[3];
+  float rgba[4];
+  bool flat;
+  fetch_slot(*D, s, node, local, cv, rgba, flat);
+  if (ent == nullptr)
+    return clipped_spans(cv, F, j, false, nullptr, nullptr, nullptr, nullptr, nullptr);
+  int lo[kMaxPoly], hi[kMaxPoly], qs[kMaxPoly];
+  float tz[kMaxPoly], zx[kMaxPoly];
+  const int c = clipped_spans(cv, F, j, true, lo, hi, tz, zx, qs);
+  for (int k = 0; k < c && k < room; ++k)
+    ent[k] = make_uint4(
+        (uint32_t)lo[k] | ((uint32_t)hi[k] << 8) | ((rref | (uint32_t)qs[k]) << 16),
+        __float_as_uint(tz[k]), __float_as_uint(zx[k]), colw);
+  return c;
+}
+
 // (a)+(b) of one slot of scene s into the scene's LDS tables.
 template <bool SMOOTH, int OM>
 __device__ __forceinline__ void setup_slot(const pbr_scene_desc& D, const RowsFrame& F,
@@ -693,11 +727,8 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
     const uint32_t fl = L.flags[r];
     int c = 0;
     if (fl & kRFlagClipped) {
-      CVert cv[3];
-      float rgba[4];
-      bool flat;
-      fetch_slot(D, s, Rr.A[0], (uint32_t)Rr.A[1], cv, rgba, flat);
-      c = clipped_spans(cv, F, j, false, nullptr, nullptr, nullptr, nullptr, nullptr);
+      c = clipped_row_entries(kernarg_scene(), F, s, Rr.A[0], (uint32_t)Rr.A[1], j, nullptr, 0,
+                              0u, 0u);
     } else {
       pbr_span::Edges E;
 #pragma unroll
@@ -746,18 +777,9 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
     if constexpr (OM == 0) colw = L.col[r];
     const RowRec& Rr = L.rec[r];
     if (L.flags[r] & kRFlagClipped) {
-      int lo[7], hi[7], qs[7];
-      float tz[7], zx[7];
-      CVert cv[3];
-      float rgba[4];
-      bool flat;
-      fetch_slot(D, s, Rr.A[0], (uint32_t)Rr.A[1], cv, rgba, flat);
-      const int c = clipped_spans(cv, F, jb0 + b, true, lo, hi, tz, zx, qs);
-      for (int k = 0; k < c && pos + (uint32_t)k < (uint32_t)kCap; ++k)
-        Q.u.ent[pos + k] = make_uint4(
-            (uint32_t)lo[k] | ((uint32_t)hi[k] << 8) |
-                ((((uint32_t)r << 3) | (uint32_t)qs[k]) << 16),
-            __float_as_uint(tz[k]), __float_as_uint(zx[k]), colw);
+      if (pos < (uint32_t)kCap)
+        clipped_row_entries(kernarg_scene(), F, s, Rr.A[0], (uint32_t)Rr.A[1], jb0 + b,
+                            &Q.u.ent[pos], kCap - (int)pos, (uint32_t)r << 3, colw);
     } else if (pos < (uint32_t)kCap) {
       const float tz = fmaf(Rr.zy, (float)(jb0 + b) + 0.5f, Rr.zc);
       Q.u.ent[pos] = make_uint4((uint32_t)plh[it] | ((uint32_t)r << 19), __float_as_uint(tz),
@@ -1039,6 +1061,8 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
   }
 }
 
+// D stays the first parameter: the out-of-line clipper reads it through the
+// kernarg segment (kernarg_scene).
 template <bool SMOOTH, int OM, int PXL, bool REF>
 __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, pbr_frame_out O,
                                                              RowsFrame F, Light lt, float4 bg) {
