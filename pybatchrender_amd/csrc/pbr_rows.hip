@@ -829,16 +829,22 @@ __device__ __forceinline__ void test_entry(const uint4& e, bool ok, int x0, cons
 
 // The lane's row-list entries [n0, n0 + cnt); `emax` (uniform) is the
 // longest list of the wave's rows.  Two entries are loaded per step (the
-// second is a no-op past the end of the list).
+// second is a no-op past the end of the list), an odd last one alone.
 template <int PXL, bool OM0, bool REF>
 __device__ __forceinline__ void walk_row(const uint4* ent, uint32_t n0, int cnt, int emax, int x0,
                                          const float* pxf, PixState<PXL>& S) {
-  for (int k = 0; k < emax; k += 2) {
+  int k = 0;
+  for (; k + 1 < emax; k += 2) {
     const bool ok0 = k < cnt, ok1 = k + 1 < cnt;
     const uint4 e0 = ent[ok0 ? min(n0 + (uint32_t)k, (uint32_t)kCap - 1u) : 0u];
     const uint4 e1 = ent[ok1 ? min(n0 + (uint32_t)k + 1u, (uint32_t)kCap - 1u) : 0u];
     test_entry<PXL, OM0, REF>(e0, ok0, x0, pxf, S);
     test_entry<PXL, OM0, REF>(e1, ok1, x0, pxf, S);
+  }
+  if (k < emax) {
+    const bool ok0 = k < cnt;
+    const uint4 e0 = ent[ok0 ? min(n0 + (uint32_t)k, (uint32_t)kCap - 1u) : 0u];
+    test_entry<PXL, OM0, REF>(e0, ok0, x0, pxf, S);
   }
 }
 
