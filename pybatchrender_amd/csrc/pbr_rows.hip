@@ -892,13 +892,15 @@ __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const Scen
   if ((F.W % PXL) == 0) {
     // whole, aligned lane pixels: one dword (or wider) store per channel
     if constexpr (OM == 0) {
-      uint8_t* dst = reinterpret_cast<uint8_t*>(so.color);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c >= so.C) break;
+      // a uniform loop over the channels (its counter and byte selector stay
+      // scalar; unrolled, the per-channel conditions were kept in spilled
+      // scalar registers)
+      uint8_t* dst = reinterpret_cast<uint8_t*>(so.color) + pix0;
+#pragma unroll 1
+      for (int c = 0; c < so.C; ++c, dst += so.plane) {
 #pragma unroll
         for (int h = 0; h < PXL; h += 4)
-          *reinterpret_cast<uint32_t*>(dst + (size_t)c * so.plane + pix0 + h) =
+          *reinterpret_cast<uint32_t*>(dst + h) =
               gather_byte(cv[h], cv[h + 1], cv[h + 2], cv[h + 3], c);
       }
     } else if constexpr (OM == 1) {
