@@ -176,10 +176,9 @@ def test_smooth_geometry_through_the_near_plane(cuda, kernel):
     check_parity(r, f"smooth near-plane clipping ({kernel})")
 
 
-@pytest.mark.parametrize("kernel", ["tiles", "chunked", "rows"])
-def test_near_plane_and_guard_band_clipping(cuda, kernel):
-    """Geometry that crosses the near plane and extends far past the view."""
-    n = 16
+def _clipping_scene(kernel: str, n: int = 16):
+    """A 4000-unit ground quad past the guard band and boxes around the camera
+    that cross the near plane; returns the renderer and the box node."""
     c = PBRConfig(num_scenes=n, tile_resolution=(64, 64), device="cuda", kernel=kernel)
     r = PBRRenderer(c)
     ground = r.add_node("models/quad", instances_per_scene=1, shared_across_scenes=True,
@@ -197,7 +196,33 @@ def test_near_plane_and_guard_band_clipping(cuda, kernel):
     r._pbr_cam.set_positions_and_lookat(eye, torch.tensor([0.0, 10.0, 0.5]))
     r.add_light()
     r.setup_environment()
+    return r, big
+
+
+@pytest.mark.parametrize("kernel", ["tiles", "chunked", "rows"])
+def test_near_plane_and_guard_band_clipping(cuda, kernel):
+    """Geometry that crosses the near plane and extends far past the view."""
+    r, _ = _clipping_scene(kernel)
     check_parity(r, f"near/far/guard clipping ({kernel})")
+
+
+def test_graph_replays_with_clipped_geometry(cuda):
+    """Replays of a captured row-span render whose triangles straddle the near
+    plane and the guard band (their row spans come from the out-of-line
+    clipper, which reads the launch's descriptor from the kernel arguments)
+    match eager renders and the oracle, with new box matrices written in
+    place between replays."""
+    r, big = _clipping_scene("rows")
+    g = r.capture_step(depth=True, vis=True)
+    gen = torch.Generator("cpu").manual_seed(5)
+    for k in range(3):
+        M = workloads.random_transforms(r.num_scenes * 2, gen, pos_range=1.0, scale_range=(1.5, 4.0))
+        big.transforms_b44.copy_(M.reshape(-1, 4, 4).to(big.transforms_b44.device))
+        got = {key: v.clone() for key, v in g().items()}
+        torch.cuda.synchronize()
+        out = check_parity(r, f"graph replay {k} with clipped geometry")
+        for key in ("pixels", "depth", "vis"):
+            assert torch.equal(_bits(got[key]), _bits(out["got"][key])), f"replay {k}: {key}"
 
 
 def test_empty_and_zero_instances(cuda):
