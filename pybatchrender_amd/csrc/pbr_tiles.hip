@@ -47,6 +47,7 @@ constexpr int kRound = kTileBlock * kPerThread;      // boxes per round
 constexpr int kQCap = 2 * kRound - 144;
 constexpr int kAutoTilePixels = 8192;                // auto tiles: 64 KiB of pixel state
 constexpr int kSmallArea = 48;   // box-in-tile pixels one lane rasterises alone
+constexpr int kResolvePx = 4;    // pixels a thread resolves per pass (flat shading)
 constexpr uint32_t kDepthOne = 0x3f800000u;
 
 struct TileFrame {
@@ -416,8 +417,11 @@ __device__ __forceinline__ void load_grec(const GRec* g, GRec& r) {
 }
 
 // Dynamic LDS: pixel keys [th][tw], then the small and large queues.
+// Flat-shaded variants are held to 128 VGPRs: two 512-thread workgroups
+// share a CU (their LDS allows it).
 template <bool SMOOTH, int CMODE>
-__global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc D,
+__global__ __launch_bounds__(kTileBlock)
+__attribute__((amdgpu_waves_per_eu(SMOOTH ? 1 : 4))) void tile_raster_kernel(pbr_scene_desc D,
                                                                  pbr_frame_out O, TileFrame F,
                                                                  TilePool P, Light lt, float4 bg,
                                                                  uint32_t total_items,
@@ -545,82 +549,151 @@ __global__ __launch_bounds__(kTileBlock) void tile_raster_kernel(pbr_scene_desc 
     // resolve: shade each pixel's winner, write the tile
     const int C = O.channels;
     const size_t plane = (size_t)H * (size_t)W;
-    for (int p = tid; p < t.w * t.h; p += kTileBlock) {
-      const int ly = p / t.w, lx = p - ly * t.w;
-      const unsigned long long v = pix[ly * F.tw + lx];
-      const uint32_t zb = (uint32_t)(v >> 32);
-      const uint32_t rank = (uint32_t)v;
-      const bool covered = zb < kDepthOne;
-      const int x = t.x0 + lx, yo = t.yo0 + ly;
-      const uint32_t sl = rank >> 3;
-      const size_t g = (size_t)t.off + sl;
-      float col[4] = {bgc[0], bgc[1], bgc[2], bgc[3]};
-      uint32_t packed = 0;
-      bool have_packed = false;
-      if (covered) {
-        bool smooth = false;
-        if (SMOOTH) smooth = (P.recs[g].flags & kFlagSmooth) != 0u;
-        if (!smooth) {
-          if (CMODE == 0) {
-            packed = P.col8[g];
-            have_packed = true;
-          } else {
-            const float4 c4 = *reinterpret_cast<const float4*>(P.shades + g * F.shw);
-            col[0] = c4.x;
-            col[1] = c4.y;
-            col[2] = c4.z;
-            col[3] = c4.w;
+    if constexpr (!SMOOTH) {
+      // flat shading: a thread takes kResolvePx pixels per pass (p,
+      // p + kTileBlock, ...); their keys are read and their colours gathered
+      // before any of them is written, so the gathers of a pass are in
+      // flight together.  p -> (p / w, p % w) from a float reciprocal: p <
+      // 2^24 and w < 2^16, so the truncated quotient is off by at most one
+      // and one step corrects it.
+      const int npx = t.w * t.h;
+      const float inv_tw = 1.0f / (float)t.w;
+      for (int p0 = tid; p0 < npx; p0 += kResolvePx * kTileBlock) {
+        unsigned long long key[kResolvePx];
+        int lyx[kResolvePx];
+        uint32_t pre[kResolvePx];  // CMODE 0: packed colour
+        float4 pre4[kResolvePx];   // CMODE 1, 2: shade
+#pragma unroll
+        for (int k = 0; k < kResolvePx; ++k) {
+          const int p = p0 + k * kTileBlock;
+          int ly = (int)((float)p * inv_tw), lx = p - ly * t.w;
+          if (lx < 0) {
+            --ly;
+            lx += t.w;
+          } else if (lx >= t.w) {
+            ++ly;
+            lx -= t.w;
           }
-        } else {
-          GRec R;
-          load_grec(P.recs + g, R);
-          const int jg = H - 1 - yo;
-          const double px = (double)(256 * x + 128), py = (double)(256 * jg + 128);
-          if (R.flags & kFlagClipped) {
-            FixTri T;
-            CVert sub[3];
-            if (clipped_sub(D, F, t.s, base, sl, (int)(rank & 7u) + 1, T, sub)) {
-              double A[3], B[3], Cc[3];
-              edges_from_fixed(T.X, T.Y, T.bias, A, B, Cc);
-              float sh[16];
-              CVert cv[3];
-              float rgba[4];
-              bool flat;
-              fetch_triangle(D, t.s, base, sl, cv, rgba, flat);
+          lyx[k] = (ly << 16) | lx;
+          key[k] = p < npx ? pix[ly * F.tw + lx] : ((unsigned long long)kDepthOne << 32);
+        }
 #pragma unroll
-              for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < kResolvePx; ++k) {
+          const size_t g = (size_t)t.off + ((uint32_t)key[k] >> 3);
+          pre[k] = 0u;
+          pre4[k] = make_float4(bgc[0], bgc[1], bgc[2], bgc[3]);
+          if ((uint32_t)(key[k] >> 32) < kDepthOne) {
+            if (CMODE == 0)
+              pre[k] = P.col8[g];
+            else
+              pre4[k] = *reinterpret_cast<const float4*>(P.shades + g * F.shw);
+          }
+        }
 #pragma unroll
-                for (int c = 0; c < 3; ++c) sh[3 * k + c] = sub[k].n[c];
-                sh[9 + k] = T.invw[k];
-              }
-#pragma unroll
-              for (int c = 0; c < 4; ++c) sh[12 + c] = rgba[c];
-              shade_smooth(A, B, Cc, T.bias, sh, px, py, lt, col);
+        for (int k = 0; k < kResolvePx; ++k) {
+          if (p0 + k * kTileBlock >= npx) break;
+          const int ly = lyx[k] >> 16, lx = lyx[k] & 0xffff;
+          const uint32_t zb = (uint32_t)(key[k] >> 32);
+          const bool covered = zb < kDepthOne;
+          const uint32_t sl = (uint32_t)key[k] >> 3;
+          const uint32_t packed = covered ? pre[k] : pack_rgba8(bgc);
+          const float col[4] = {pre4[k].x, pre4[k].y, pre4[k].z, pre4[k].w};
+          const size_t pidx = (size_t)(t.yo0 + ly) * W + (t.x0 + lx);
+          for (int c = 0; c < C; ++c) {
+            const size_t o = ((size_t)t.s * C + c) * plane + pidx;
+            if (O.dtype == PBR_DTYPE_U8) {
+              reinterpret_cast<uint8_t*>(O.color)[o] = (uint8_t)((packed >> (8 * c)) & 255u);
+            } else {
+              const float vv = col[c];
+              if (O.dtype == PBR_DTYPE_F32)
+                reinterpret_cast<float*>(O.color)[o] = vv;
+              else
+                reinterpret_cast<uint16_t*>(O.color)[o] = (uint16_t)bf16_rne(vv);
+            }
+          }
+          const size_t dofs = (size_t)t.s * plane + pidx;
+          if (O.depth) O.depth[dofs] = covered ? __uint_as_float(zb) : 1.0f;
+          if (O.vis) O.vis[dofs] = covered ? (int32_t)sl : -1;
+        }
+      }
+    } else {
+      for (int p = tid; p < t.w * t.h; p += kTileBlock) {
+        const int ly = p / t.w, lx = p - ly * t.w;
+        const unsigned long long v = pix[ly * F.tw + lx];
+        const uint32_t zb = (uint32_t)(v >> 32);
+        const uint32_t rank = (uint32_t)v;
+        const bool covered = zb < kDepthOne;
+        const int x = t.x0 + lx, yo = t.yo0 + ly;
+        const uint32_t sl = rank >> 3;
+        const size_t g = (size_t)t.off + sl;
+        float col[4] = {bgc[0], bgc[1], bgc[2], bgc[3]};
+        uint32_t packed = 0;
+        bool have_packed = false;
+        if (covered) {
+          bool smooth = false;
+          if (SMOOTH) smooth = (P.recs[g].flags & kFlagSmooth) != 0u;
+          if (!smooth) {
+            if (CMODE == 0) {
+              packed = P.col8[g];
+              have_packed = true;
+            } else {
+              const float4 c4 = *reinterpret_cast<const float4*>(P.shades + g * F.shw);
+              col[0] = c4.x;
+              col[1] = c4.y;
+              col[2] = c4.z;
+              col[3] = c4.w;
             }
           } else {
-            double A[3], B[3], Cc[3];
-            edges_from_fixed(R.X, R.Y, R.flags & 7u, A, B, Cc);
-            shade_smooth(A, B, Cc, R.flags & 7u, P.shades + g * F.shw, px, py, lt, col);
+            GRec R;
+            load_grec(P.recs + g, R);
+            const int jg = H - 1 - yo;
+            const double px = (double)(256 * x + 128), py = (double)(256 * jg + 128);
+            if (R.flags & kFlagClipped) {
+              FixTri T;
+              CVert sub[3];
+              if (clipped_sub(D, F, t.s, base, sl, (int)(rank & 7u) + 1, T, sub)) {
+                double A[3], B[3], Cc[3];
+                edges_from_fixed(T.X, T.Y, T.bias, A, B, Cc);
+                float sh[16];
+                CVert cv[3];
+                float rgba[4];
+                bool flat;
+                fetch_triangle(D, t.s, base, sl, cv, rgba, flat);
+  #pragma unroll
+                for (int k = 0; k < 3; ++k) {
+  #pragma unroll
+                  for (int c = 0; c < 3; ++c) sh[3 * k + c] = sub[k].n[c];
+                  sh[9 + k] = T.invw[k];
+                }
+  #pragma unroll
+                for (int c = 0; c < 4; ++c) sh[12 + c] = rgba[c];
+                shade_smooth(A, B, Cc, T.bias, sh, px, py, lt, col);
+              }
+            } else {
+              double A[3], B[3], Cc[3];
+              edges_from_fixed(R.X, R.Y, R.flags & 7u, A, B, Cc);
+              shade_smooth(A, B, Cc, R.flags & 7u, P.shades + g * F.shw, px, py, lt, col);
+            }
           }
         }
-      }
-      if (CMODE == 0 && !have_packed) packed = pack_rgba8(col);
-      const size_t pidx = (size_t)yo * W + x;
-      for (int c = 0; c < C; ++c) {
-        const size_t o = ((size_t)t.s * C + c) * plane + pidx;
-        if (O.dtype == PBR_DTYPE_U8) {
-          reinterpret_cast<uint8_t*>(O.color)[o] = (uint8_t)((packed >> (8 * c)) & 255u);
-        } else {
-          const float vv = col[c];
-          if (O.dtype == PBR_DTYPE_F32)
-            reinterpret_cast<float*>(O.color)[o] = vv;
-          else
-            reinterpret_cast<uint16_t*>(O.color)[o] = (uint16_t)bf16_rne(vv);
+        if (CMODE == 0 && !have_packed) packed = pack_rgba8(col);
+        const size_t pidx = (size_t)yo * W + x;
+        for (int c = 0; c < C; ++c) {
+          const size_t o = ((size_t)t.s * C + c) * plane + pidx;
+          if (O.dtype == PBR_DTYPE_U8) {
+            reinterpret_cast<uint8_t*>(O.color)[o] = (uint8_t)((packed >> (8 * c)) & 255u);
+          } else {
+            const float vv = col[c];
+            if (O.dtype == PBR_DTYPE_F32)
+              reinterpret_cast<float*>(O.color)[o] = vv;
+            else
+              reinterpret_cast<uint16_t*>(O.color)[o] = (uint16_t)bf16_rne(vv);
+          }
         }
+        const size_t dofs = (size_t)t.s * plane + pidx;
+        if (O.depth) O.depth[dofs] = covered ? __uint_as_float(zb) : 1.0f;
+        if (O.vis) O.vis[dofs] = covered ? (int32_t)sl : -1;
       }
-      const size_t dofs = (size_t)t.s * plane + pidx;
-      if (O.depth) O.depth[dofs] = covered ? __uint_as_float(zb) : 1.0f;
-      if (O.vis) O.vis[dofs] = covered ? (int32_t)sl : -1;
     }
 
     if (!persistent) break;
