@@ -178,6 +178,16 @@ __device__ __forceinline__ void clip_class(const CVert cv[3], float g, bool& all
 
 // Exclusive prefix of per-node triangle counts for scene s (node.py:45-46:
 // cnt instances of the node's mesh, counts[] clamped to [0, ips]).
+// The launch's scene descriptor where the kernel arguments lie, as a plain
+// pointer for out-of-line code.  Valid in kernels whose first parameter is
+// the pbr_scene_desc (it then starts the kernarg segment).  Taking the
+// address of the by-value argument instead makes every lane copy the whole
+// descriptor into private memory at kernel entry.
+__device__ __forceinline__ const pbr_scene_desc* kernarg_scene() {
+  return reinterpret_cast<const pbr_scene_desc*>(
+      (const void*)__builtin_amdgcn_kernarg_segment_ptr());
+}
+
 __device__ void node_table(const pbr_scene_desc& D, int s, uint32_t* base) {
   uint32_t acc = 0;
   for (int j = 0; j < D.num_nodes; ++j) {

@@ -347,15 +347,6 @@ __device__ __forceinline__ void edges_of(const FixTri& T, RowRec& R) {
   R.zc = T.zc;
 }
 
-// The launch's scene descriptor where the kernel arguments lie (it is the
-// first argument of rows_kernel, so it starts the kernarg segment), as a
-// plain pointer for out-of-line code: taking the address of the by-value
-// argument would make each lane copy it into private memory first.
-__device__ __forceinline__ const pbr_scene_desc* kernarg_scene() {
-  return reinterpret_cast<const pbr_scene_desc*>(
-      (const void*)__builtin_amdgcn_kernarg_segment_ptr());
-}
-
 // Row-list entries at GL row j of the clipped triangle (node, local) of
 // scene s: their number, and with `ent` set the entries themselves
 // (lo | hi << 8 | (rref | sub-triangle) << 16, tz, zx, colw), at most

@@ -370,12 +370,12 @@ __device__ __forceinline__ bool clipped_sub(const pbr_scene_desc& D, const TileF
 // Every fragment-producing sub-triangle of a clipped slot; `lane` < 0 means
 // one lane does all the pixels, otherwise the wave shares them.
 __device__ __noinline__ void raster_clipped(unsigned long long* pix, const TileCtx& t,
-                                            const pbr_scene_desc& D, const TileFrame& F,
+                                            const pbr_scene_desc* D, TileFrame F,
                                             const uint32_t* base, uint32_t slot, int lane) {
   CVert cv[3];
   float rgba[4];
   bool flat;
-  fetch_triangle(D, t.s, base, slot, cv, rgba, flat);
+  fetch_triangle(*D, t.s, base, slot, cv, rgba, flat);
   CVert poly[kMaxPoly];
   poly[0] = cv[0];
   poly[1] = cv[1];
@@ -418,7 +418,8 @@ __device__ __forceinline__ void load_grec(const GRec* g, GRec& r) {
 
 // Dynamic LDS: pixel keys [th][tw], then the small and large queues.
 // Flat-shaded variants are held to 128 VGPRs: two 512-thread workgroups
-// share a CU (their LDS allows it).
+// share a CU (their LDS allows it).  D stays the first parameter: the
+// out-of-line clipper reads it through the kernarg segment (kernarg_scene).
 template <bool SMOOTH, int CMODE>
 __global__ __launch_bounds__(kTileBlock)
 __attribute__((amdgpu_waves_per_eu(SMOOTH ? 1 : 4))) void tile_raster_kernel(pbr_scene_desc D,
@@ -506,7 +507,7 @@ __attribute__((amdgpu_waves_per_eu(SMOOTH ? 1 : 4))) void tile_raster_kernel(pbr
           GRec R;
           load_grec(P.recs + t.off + sl, R);
           if (R.flags & kFlagClipped)
-            raster_clipped(pix, t, D, F, base, sl, -1);
+            raster_clipped(pix, t, kernarg_scene(), F, base, sl, -1);
           else
             raster_lane(pix, t, R.X, R.Y, R.flags & 7u, R.zx, R.zy, R.zc, R.i0, R.i1, R.j0,
                         R.j1, sl << 3);
@@ -532,7 +533,7 @@ __attribute__((amdgpu_waves_per_eu(SMOOTH ? 1 : 4))) void tile_raster_kernel(pbr
           if (kn < cb) stage_grec(P.recs + t.off + qb[kn], stage[wv][buf ^ 1], lane);
           buf ^= 1;
           if (R.flags & kFlagClipped)
-            raster_clipped(pix, t, D, F, base, sl, lane);
+            raster_clipped(pix, t, kernarg_scene(), F, base, sl, lane);
           else
             raster_wave(pix, t, lane, R.X, R.Y, R.flags & 7u, R.zx, R.zy, R.zc, R.i0, R.i1,
                         R.j0, R.j1, sl << 3);
