@@ -417,12 +417,12 @@ __device__ __forceinline__ void load_grec(const GRec* g, GRec& r) {
 }
 
 // Dynamic LDS: pixel keys [th][tw], then the small and large queues.
-// Flat-shaded variants are held to 128 VGPRs: two 512-thread workgroups
-// share a CU (their LDS allows it).  D stays the first parameter: the
+// Held to 128 VGPRs, so two 512-thread workgroups share a CU (their LDS
+// allows it).  D stays the first parameter: the
 // out-of-line clipper reads it through the kernarg segment (kernarg_scene).
 template <bool SMOOTH, int CMODE>
 __global__ __launch_bounds__(kTileBlock)
-__attribute__((amdgpu_waves_per_eu(SMOOTH ? 1 : 4))) void tile_raster_kernel(pbr_scene_desc D,
+__attribute__((amdgpu_waves_per_eu(4))) void tile_raster_kernel(pbr_scene_desc D,
                                                                  pbr_frame_out O, TileFrame F,
                                                                  TilePool P, Light lt, float4 bg,
                                                                  uint32_t total_items,
