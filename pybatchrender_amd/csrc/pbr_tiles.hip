@@ -156,8 +156,11 @@ __device__ __forceinline__ uint32_t pool_total(const pbr_scene_desc& D, const Ti
   return (uint32_t)min(t, (uint64_t)F.budget);
 }
 
-// Scene s and its slot of pool slot g (binary search of the scene offsets
-// when instance counts differ per scene).
+// Scene s and its slot of pool slot g.  When instance counts differ per
+// scene, s is the largest scene with scene_off[s] <= g: on entry s >= 0 is
+// the scene of a smaller pool slot of the same thread, walked forward at
+// most four scenes before falling back to a binary search from there;
+// s < 0 searches all scenes.
 __device__ __forceinline__ void pool_locate(const pbr_scene_desc& D, const TileFrame& F,
                                             const TilePool& P, uint32_t g, int& s,
                                             uint32_t& slot) {
@@ -166,7 +169,15 @@ __device__ __forceinline__ void pool_locate(const pbr_scene_desc& D, const TileF
     slot = g - (uint32_t)s * F.max_slots;
     return;
   }
-  int lo = 0, hi = D.num_scenes;  // largest s with scene_off[s] <= g
+  int lo = 0, hi = D.num_scenes;
+  if (s >= 0) {
+    for (int k = 0; k < 4 && s + 1 < hi && P.scene_off[s + 1] <= g; ++k) ++s;
+    if (s + 1 >= hi || P.scene_off[s + 1] > g) {
+      slot = g - P.scene_off[s];
+      return;
+    }
+    lo = s;
+  }
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
     if (P.scene_off[mid] <= g) lo = mid;
@@ -245,8 +256,15 @@ template <bool SMOOTH>
 __global__ __launch_bounds__(256) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
                                                          TilePool P, Light lt) {
   const uint32_t total = pool_total(D, F, P);
-  for (uint32_t g = blockIdx.x * 256u + threadIdx.x; g < total; g += gridDim.x * 256u) {
-    int s;
+  // each workgroup takes a contiguous range of pool slots, 256 consecutive
+  // slots per step (coalesced stores), so a thread's next slot is usually in
+  // the same scene as its last
+  const uint32_t per = (uint32_t)(((((uint64_t)total + gridDim.x - 1) / gridDim.x) + 255u) &
+                                  ~(uint64_t)255u);
+  const uint64_t g_end = min((uint64_t)total, (uint64_t)(blockIdx.x + 1) * per);
+  int s = -1;
+  for (uint64_t g64 = (uint64_t)blockIdx.x * per + threadIdx.x; g64 < g_end; g64 += 256u) {
+    const uint32_t g = (uint32_t)g64;
     uint32_t slot;
     pool_locate(D, F, P, g, s, slot);
     int j;
