@@ -155,6 +155,34 @@ def test_tiles_uneven_batch_full_size_slice(cuda):
     check_parity(r, "uneven 84x84 up to 256 instances")
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_tiles_many_tiny_and_empty_scenes(cuda, persistent):
+    """Per-scene counts of 0, 1 or 2 boxes over 700 scenes, with runs of
+    empty scenes: one 256-slot setup step crosses many scenes (and every
+    scene boundary of a run of zeros), so the tile setup's walk from the
+    previous slot's scene and its binary-search fallback both run."""
+    n = 700
+    c = PBRConfig(num_scenes=n, tile_resolution=(24, 24), device="cuda", kernel="tiles",
+                  persistent=persistent)
+    r = PBRRenderer(c)
+    # 8 boxes (96 triangle slots) per scene at most; counts leave most of
+    # them unused
+    box = r.add_node("models/box", instances_per_scene=8, model_pivot_relative_point=(0.5, 0.5, 0.5))
+    gen = torch.Generator("cpu").manual_seed(11)
+    counts = torch.randint(0, 3, (n,), generator=gen, dtype=torch.int32)
+    counts[100:180] = 0                     # a run of empty scenes
+    counts[400:420] = 8                     # a few full ones
+    box.set_instance_counts(counts)
+    box.set_transforms(workloads.random_transforms(n * 8, gen, pos_range=1.0, scale_range=(0.5, 1.2)))
+    box.set_colors(torch.rand(n * 8, 4, generator=gen))
+    r.add_camera(fov_y_deg=60.0, z_near=0.1, z_far=50.0)
+    r._pbr_cam.set_positions_and_lookat(torch.tensor([[0.0, -5.0, 1.0]]).repeat(n, 1),
+                                        torch.tensor([0.0, 0.0, 0.0]))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, f"700 tiny and empty scenes (persistent={persistent})")
+
+
 @pytest.mark.parametrize("kernel", ["tiles", "chunked"])
 def test_smooth_geometry_through_the_near_plane(cuda, kernel):
     """Spheres and boxes straddling the near plane: clipped smooth sub-triangles."""
