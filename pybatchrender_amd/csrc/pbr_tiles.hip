@@ -259,8 +259,7 @@ __global__ __launch_bounds__(256) void tile_setup_kernel(pbr_scene_desc D, TileF
   // each workgroup takes a contiguous range of pool slots, 256 consecutive
   // slots per step (coalesced stores), so a thread's next slot is usually in
   // the same scene as its last
-  const uint32_t per = (uint32_t)(((((uint64_t)total + gridDim.x - 1) / gridDim.x) + 255u) &
-                                  ~(uint64_t)255u);
+  const uint64_t per = ((((uint64_t)total + gridDim.x - 1) / gridDim.x) + 255u) & ~(uint64_t)255u;
   const uint64_t g_end = min((uint64_t)total, (uint64_t)(blockIdx.x + 1) * per);
   int s = -1;
   for (uint64_t g64 = (uint64_t)blockIdx.x * per + threadIdx.x; g64 < g_end; g64 += 256u) {
