@@ -94,7 +94,7 @@ def triangles_drawn(r) -> int:
 
 def config_renderer(name: str, persistent: bool):
     """The renderer of a named workload for --kernel-only timing."""
-    from pybatchrender_amd import workloads
+    from pybatchrender import workloads
 
     if name == "many-cubes":
         return workloads.many_cubes(persistent=persistent)
@@ -110,7 +110,7 @@ def kernel_time_us(r, launches: int = 200) -> float:
     launches of one prebuilt descriptor on the current stream."""
     import torch
 
-    from pybatchrender_amd import _abi, _native
+    from pybatchrender import _abi, _native
 
     N, C, H, W = r.output_shape()
     tdt, code = {"uint8": (torch.uint8, _abi.PBR_DTYPE_U8),
@@ -170,11 +170,11 @@ def fwd_bwd_leg(n: int, state, launches: int = 100) -> dict:
     plain forward (colour only) vs forward with the visibility buffer plus
     pbr_render_backward (gradients for instance colours, light and vertices),
     all through the C ABI on one stream; and the same pair through autograd
-    (pybatchrender_amd.diff)."""
+    (pybatchrender.diff)."""
     import torch
 
-    from pybatchrender_amd import _abi, _native, diff
-    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender import _abi, _native, diff
+    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
 
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda",
                                               output_dtype="float32"))
@@ -266,8 +266,8 @@ def env_loop_leg(n: int = 16384, steps: int = 1000) -> dict:
     HIP events between consecutive replays; throughput from the wall clock."""
     import torch
 
-    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
-    from pybatchrender_amd.envs.cartpole.loop import CartPoleLoop
+    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender.envs.cartpole.loop import CartPoleLoop
 
     cfg = CartPoleConfig(num_scenes=n, device="cuda", output_dtype="bfloat16", output_depth=True,
                          theta_threshold_deg=math.degrees(0.21), x_threshold=2.4, max_steps=500,
@@ -382,7 +382,7 @@ def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 8.0,
     import numpy as np
     import torch
 
-    from pybatchrender_amd import _abi
+    from pybatchrender import _abi
 
     N, C, H, W = sample_scenes, 3, hs.desc.height, hs.desc.width
     pix = torch.empty((N, C, H, W), dtype=torch.uint8)
@@ -405,7 +405,7 @@ def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 8.0,
 def env_step_rate(n: int, steps: int) -> float:
     import torch
 
-    import pybatchrender_amd as pbr
+    import pybatchrender as pbr
 
     env = pbr.envs.make("CartPole-v0", num_scenes=n)
     td = env.reset()
@@ -459,7 +459,7 @@ def main() -> int:
 
     import torch
 
-    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
 
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)

@@ -28,7 +28,7 @@ from __future__ import annotations
 
 import torch
 
-from pybatchrender_amd import _abi
+from pybatchrender import _abi
 
 
 class DenseScene:

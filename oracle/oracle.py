@@ -17,7 +17,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
-from pybatchrender_amd import _abi
+from pybatchrender import _abi
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle_pbr.so"

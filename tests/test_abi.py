@@ -5,8 +5,8 @@ from pathlib import Path
 
 import pytest
 
-import pybatchrender_amd as pbr
-from pybatchrender_amd import _abi, _native
+import pybatchrender as pbr
+from pybatchrender import _abi, _native
 
 ROOT = Path(__file__).resolve().parent.parent
 

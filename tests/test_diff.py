@@ -13,9 +13,9 @@ import torch
 
 from oracle import oracle
 from oracle.diff_ref import DenseScene, interior_mask, loss_fn
-from pybatchrender_amd import PBRConfig, PBRRenderer, diff
-from pybatchrender_amd.renderer.mesh import uv_sphere
-from pybatchrender_amd.workloads import random_transforms
+from pybatchrender import PBRConfig, PBRRenderer, diff
+from pybatchrender.renderer.mesh import uv_sphere
+from pybatchrender.workloads import random_transforms
 
 
 def mixed_scene(device: str, N: int = 3, res: int = 48, seed: int = 0, channels: int = 3,
@@ -250,7 +250,7 @@ def test_backward_strength_clamp_and_uneven_counts(cuda):
 
 @pytest.mark.gpu
 def test_backward_cartpole_ground_scene(cuda):
-    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=16, device="cuda",
                                               output_dtype="float32"))
     g = torch.Generator("cpu").manual_seed(0)
@@ -285,7 +285,7 @@ def test_backward_flat_uneven_counts(cuda):
 def _abi_backward(r: PBRRenderer, gout: torch.Tensor, rows: int | None = None) -> dict:
     """pbr_render_backward through the C ABI with a workspace of `rows`
     partial rows (None: the size pbr_backward_workspace_bytes asks for)."""
-    from pybatchrender_amd import _abi, _native
+    from pybatchrender import _abi, _native
     p = diff.scene_params(r, requires_grad=False)
     desc, keep = diff._descriptor(r, p.vertices, p.light, p.colors)
     vis = r.render(vis=True)["vis"].contiguous()

@@ -10,9 +10,9 @@ import torch
 
 from oracle import cartpole_ref as ref
 from oracle import oracle
-from pybatchrender_amd import _abi, _native
-from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
-from pybatchrender_amd.envs.cartpole.loop import CartPoleLoop, cartpole_params
+from pybatchrender import _abi, _native
+from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+from pybatchrender.envs.cartpole.loop import CartPoleLoop, cartpole_params
 
 
 def _params(**kw) -> dict:

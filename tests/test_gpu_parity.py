@@ -16,8 +16,8 @@ import pytest
 import torch
 
 from oracle import oracle
-from pybatchrender_amd import PBRConfig, PBRRenderer, workloads
-from pybatchrender_amd.envs.cartpole import (CartPoleConfig, CartPoleGroundRenderer,
+from pybatchrender import PBRConfig, PBRRenderer, workloads
+from pybatchrender.envs.cartpole import (CartPoleConfig, CartPoleGroundRenderer,
                                              CartPoleRenderer)
 
 pytestmark = pytest.mark.gpu
@@ -33,17 +33,28 @@ def _bits(t: torch.Tensor) -> torch.Tensor:
     return t
 
 
-def check_parity(r: PBRRenderer, label: str) -> dict:
+def check_parity(r: PBRRenderer, label: str, depth: bool = True, vis: bool = True,
+                 got: dict | None = None) -> dict:
+    """Render (or take `got`, a finished render of the current state) and
+    compare with the oracle bitwise.  depth/vis False request no such output,
+    which selects the kernel variants that do not carry them."""
     print(f"[parity] {label}: N={r.num_scenes} {r.cfg.tile_resolution} {r.cfg.output_dtype} "
-          f"C={r.cfg.num_channels}", flush=True)
-    got = r.render(depth=True, vis=True)
+          f"C={r.cfg.num_channels} depth={depth} vis={vis}", flush=True)
+    if got is None:
+        got = r.render(depth=depth, vis=vis)
     torch.cuda.synchronize()
     desc, keep = r.scene_descriptor()
     ref = oracle.render(desc, keep, channels=int(r.cfg.num_channels), dtype=r.cfg.output_dtype)
-    gv, rv = got["vis"].cpu(), ref["vis"]
-    bad = (gv != rv).sum().item()
-    assert bad == 0, f"{label}: {bad} visibility mismatches"
-    assert torch.equal(_bits(got["depth"]), _bits(ref["depth"])), f"{label}: depth differs"
+    if vis:
+        gv, rv = got["vis"].cpu(), ref["vis"]
+        bad = (gv != rv).sum().item()
+        assert bad == 0, f"{label}: {bad} visibility mismatches"
+    else:
+        assert "vis" not in got
+    if depth:
+        assert torch.equal(_bits(got["depth"]), _bits(ref["depth"])), f"{label}: depth differs"
+    else:
+        assert "depth" not in got
     gp, rp = _bits(got["pixels"]), _bits(ref["pixels"])
     assert torch.equal(gp, rp), f"{label}: {(gp != rp).sum().item()} pixel values differ"
     return {"got": got, "ref": ref}
@@ -284,7 +295,7 @@ def test_unlit_without_light(cuda):
 
 
 def test_env_step_renders(cuda):
-    import pybatchrender_amd as pbr
+    import pybatchrender as pbr
 
     env = pbr.envs.make("CartPole-v0", num_scenes=64)
     td = env.reset()
@@ -320,7 +331,7 @@ def test_scene_kernel_bins(cuda, bw, per):
 
 @pytest.mark.parametrize("dtype,channels", [("float32", 4), ("bfloat16", 3), ("uint8", 2)])
 def test_scene_kernel_smooth_dtypes(cuda, dtype, channels):
-    from pybatchrender_amd.renderer import mesh
+    from pybatchrender.renderer import mesh
 
     low = mesh.uv_sphere(segments=6, rings=4)  # 36 smooth triangles
     r = _small_scene(24, (40, 40), 22, model=low, instances=1, kernel="scene",
@@ -377,7 +388,7 @@ def test_rows_kernel_short_lists(cuda, cap):
 @pytest.mark.parametrize("dtype,channels", [("float32", 4), ("bfloat16", 3), ("uint8", 2),
                                             ("uint8", 1)])
 def test_rows_kernel_smooth_dtypes(cuda, dtype, channels):
-    from pybatchrender_amd.renderer import mesh
+    from pybatchrender.renderer import mesh
 
     low = mesh.uv_sphere(segments=6, rings=4)  # 36 smooth triangles
     r = _small_scene(24, (40, 40), 33, model=low, instances=1, kernel="rows",
@@ -434,7 +445,7 @@ def test_rows_kernel_heavy_clipping(cuda, cap):
 def test_rows_kernel_smooth_through_the_near_plane(cuda):
     """Low-poly spheres around the camera: clipped smooth sub-triangles are
     shaded from their re-derived vertices."""
-    from pybatchrender_amd.renderer import mesh
+    from pybatchrender.renderer import mesh
 
     n = 10
     low = mesh.uv_sphere(segments=6, rings=4)
@@ -459,7 +470,7 @@ def test_rows_kernel_per_triangle_setup(cuda):
     """Meshes with many more vertices than their triangles use (here 400
     vertices, 12 triangles) overflow the vertex table; such batches set up
     each triangle on its own lane."""
-    from pybatchrender_amd.renderer import mesh
+    from pybatchrender.renderer import mesh
 
     box = mesh.box()
     pad = 400 - box.num_vertices
@@ -550,7 +561,7 @@ def test_batched_poses_match_single(cuda):
 
 
 def test_autotune_picks_a_shape_that_matches_the_oracle(cuda, tmp_path):
-    from pybatchrender_amd import autotune
+    from pybatchrender import autotune
     r = workloads.uneven_batch(num_scenes=48, resolution=84, max_instances=32, seed=21)
     cache = tmp_path / "tune.json"
     rep = autotune.autotune(r, launches=2, cache_path=cache)
@@ -563,7 +574,7 @@ def test_autotune_picks_a_shape_that_matches_the_oracle(cuda, tmp_path):
 
 def test_meshes_loaded_from_files(cuda, tmp_path):
     """Nodes built from an egg and a glTF file on disk render like any mesh."""
-    from pybatchrender_amd.renderer import mesh as meshmod
+    from pybatchrender.renderer import mesh as meshmod
     from tests.test_models import _quad_gltf, write_egg
     egg = write_egg(meshmod.cone(16), tmp_path / "cone.egg")
     gltf = _quad_gltf(tmp_path, double_sided=True)

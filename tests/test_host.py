@@ -14,10 +14,10 @@ import numpy as np
 import pytest
 import torch
 
-import pybatchrender_amd as pbr
-from pybatchrender_amd import PBRConfig, PBRRenderer
-from pybatchrender_amd.renderer import mesh as meshes
-from pybatchrender_amd.renderer.shader_context import PBRShaderContext
+import pybatchrender as pbr
+from pybatchrender import PBRConfig, PBRRenderer
+from pybatchrender.renderer import mesh as meshes
+from pybatchrender.renderer.shader_context import PBRShaderContext
 
 
 # --- config -----------------------------------------------------------------------
@@ -267,8 +267,8 @@ def test_cartpole_dynamics_closed_form():
 
 
 def test_autotune_candidates_and_signature():
-    from pybatchrender_amd import autotune
-    from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender import autotune
+    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
 
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=8, device="cpu"))
     cands = autotune.candidates(r)

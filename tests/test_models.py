@@ -10,8 +10,8 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from pybatchrender_amd.renderer import mesh as meshmod
-from pybatchrender_amd.renderer.model_io import load_egg, load_gltf
+from pybatchrender.renderer import mesh as meshmod
+from pybatchrender.renderer.model_io import load_egg, load_gltf
 
 REF_MODELS = Path("/root/reference/pybatchrender/models")
 

@@ -20,8 +20,8 @@ from hypothesis import given, settings
 from hypothesis import strategies as st
 
 from oracle import oracle
-from pybatchrender_amd import PBRConfig, PBRRenderer
-from pybatchrender_amd.envs.cartpole import CartPoleConfig, CartPoleRenderer
+from pybatchrender import PBRConfig, PBRRenderer
+from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleRenderer
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 

@@ -6,9 +6,9 @@ from __future__ import annotations
 import pytest
 import torch
 
-import pybatchrender_amd as pbr
+import pybatchrender as pbr
 from oracle import steering_ref
-from pybatchrender_amd.envs.steering import SteeringConfig, SteeringEnv, SteeringRenderer
+from pybatchrender.envs.steering import SteeringConfig, SteeringEnv, SteeringRenderer
 
 
 def _params(env) -> dict:
