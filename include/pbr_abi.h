@@ -206,6 +206,24 @@ int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out,
                const pbr_render_opts* opts, void* workspace,
                size_t workspace_bytes, void* stream);
 
+/* Scratch memory of pbr_render_noalloc per GPU: 64 MiB. */
+#define PBR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* pbr_render with no workspace argument and no allocation.  No counterpart
+ * in the reference; it serves callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The kernels' scratch memory (the tile path's record pool and
+ * work queue, the scene kernel's records, the chunked kernel's queue
+ * counter) is the library's own static device array of
+ * PBR_NOALLOC_SCRATCH_BYTES on the current device; a batch whose
+ * pbr_workspace_bytes exceeds that fails with PBR_ERR_WORKSPACE before
+ * anything is enqueued.  Outputs are the same as pbr_render's.  Every call
+ * on one GPU uses that GPU's one array: calls that overlap in time on one
+ * GPU (on different streams, or from different host threads) race.  The
+ * stream must belong to the current device. */
+int pbr_render_noalloc(const pbr_scene_desc* scene, const pbr_frame_out* out,
+                       const pbr_render_opts* opts, void* stream);
+
 /* Write `rows` instance matrices from per-component pose sources on
  * `stream`.  Replaces the host path of node.py:116-154 (set_positions,
  * set_hprs, set_scales, _upload_current_transforms) and the CartPole state

@@ -11,6 +11,7 @@ PBR_ABI_VERSION = 3
 PBR_MAX_NODES = 16
 PBR_MAX_MESHES = 16
 PBR_MAX_RESOLUTION = 4096
+PBR_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 PBR_OK = 0
 PBR_ERR_INVALID = 1
@@ -204,6 +205,7 @@ def expected_layout() -> list[int]:
 # Exported symbols of libpbr_hip.so, as declared in include/pbr_abi.h.
 EXPORTS = (
     "pbr_render",
+    "pbr_render_noalloc",
     "pbr_render_backward",
     "pbr_backward_workspace_bytes",
     "pbr_set_pose",
@@ -228,6 +230,13 @@ def declare(lib: C.CDLL) -> C.CDLL:
         C.c_void_p,
     ]
     lib.pbr_render.restype = C.c_int
+    lib.pbr_render_noalloc.argtypes = [
+        C.POINTER(pbr_scene_desc),
+        C.POINTER(pbr_frame_out),
+        C.POINTER(pbr_render_opts),
+        C.c_void_p,
+    ]
+    lib.pbr_render_noalloc.restype = C.c_int
     lib.pbr_render_backward.argtypes = [
         C.POINTER(pbr_scene_desc),
         C.c_void_p,

@@ -38,6 +38,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "../../include/pbr_abi.h"
 #include "pbr_device.h"
 #include "pbr_host.h"
@@ -1099,6 +1101,40 @@ int launch_small(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rend
               px);
 }
 
+// --- pbr_render_noalloc: the library's own scratch memory -------------------
+// Each GPU the library runs on holds one copy of `scratch` (a code-object
+// variable, allocated when the code object loads on that device) until the
+// process ends.  In host code `scratch` names a host-side shadow, not the
+// device memory: the device address comes from hipGetSymbolAddress, taken
+// once per device with that device current, and only that address is
+// passed to kernels.
+static __device__ __align__(256) unsigned char scratch[PBR_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices];
+
+// The device address of `scratch` on the current device.
+int scratch_on_current_device(void** ptr) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "hipGetDevice failed: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(PBR_ERR_UNSUPPORTED, "device %d: scratch memory is kept for devices 0..%d", dev,
+                kMaxScratchDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PBR_ERR_LAUNCH, "device %d: address of the scratch array unavailable: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *ptr = g_scratch_dev[dev];
+  return PBR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1208,6 +1244,24 @@ int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out, const pbr_
     default:
       return launch_chunked(scene, out, opts, workspace, workspace_bytes, st);
   }
+}
+
+int pbr_render_noalloc(const pbr_scene_desc* scene, const pbr_frame_out* out,
+                       const pbr_render_opts* opts, void* stream) {
+  int rc = validate(scene, out);
+  if (rc) return rc;
+  size_t need = 0;
+  rc = pbr_workspace_bytes(scene, opts, &need);
+  if (rc) return rc;
+  if (need > PBR_NOALLOC_SCRATCH_BYTES)
+    return fail(PBR_ERR_WORKSPACE,
+                "batch needs %zu bytes of scratch memory; pbr_render_noalloc holds %zu (use "
+                "pbr_render with a workspace of pbr_workspace_bytes)",
+                need, (size_t)PBR_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  return pbr_render(scene, out, opts, ws, PBR_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 }  // extern "C"
