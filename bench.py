@@ -25,6 +25,10 @@ Printed JSON line (one, from this single process):
                forward with the visibility buffer + pbr_render_backward
                (colour, light and vertex gradients), C-ABI launches timed
                with HIP events, and the autograd step through diff.render
+  fwd_bwd_differentiable  the same pair on the differentiable-render
+               workload: 512 scenes 128x128 float32, 16 boxes + 16 UV
+               spheres per scene, backward of an L2 loss to a target image
+               rendered from perturbed parameters
   env_loop     16384-scene on-device CartPole loop (pbr_cartpole_step +
                render, bf16 RGB + depth), one hipGraph replay per step
   env_step     CartPole-v0 (reference scene) env.step at the same size:
@@ -165,20 +169,15 @@ def time_launches_us(launch, launches: int = 200, warm: int = 20) -> float:
     return e0.elapsed_time(e1) * 1000.0 / launches
 
 
-def fwd_bwd_leg(n: int, state, launches: int = 100) -> dict:
-    """Differentiable render of the same workload with float32 output:
-    plain forward (colour only) vs forward with the visibility buffer plus
-    pbr_render_backward (gradients for instance colours, light and vertices),
-    all through the C ABI on one stream; and the same pair through autograd
-    (pybatchrender.diff)."""
+def _fwd_bwd_launch_times(r, gout, launches: int) -> dict:
+    """Average durations (us) of the plain float32 forward (colour only), the
+    forward with the visibility buffer and pbr_render_backward of `gout`
+    (gradients for every node's colours, the light and the vertex pool), all
+    C-ABI launches back to back on one stream, timed with HIP events."""
     import torch
 
-    from pybatchrender import _abi, _native, diff
-    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender import _abi, _native
 
-    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda",
-                                              output_dtype="float32"))
-    r.step(state)
     N, C, H, W = r.output_shape()
     desc, keep = r.scene_descriptor()
     opts = r.render_opts()
@@ -207,7 +206,7 @@ def fwd_bwd_leg(n: int, state, launches: int = 100) -> dict:
     for j, t in enumerate(gcols):
         g.colors[j] = t.data_ptr()
     g.vertices, g.light = gverts.data_ptr(), glight.data_ptr()
-    gout = torch.randn((N, C, H, W), device="cuda")
+    gout = gout.to(torch.float32).contiguous()
     bws_bytes = _native.backward_workspace_bytes(desc, C, g)
     bws = torch.empty(max(bws_bytes, 1), dtype=torch.uint8, device="cuda")
 
@@ -221,42 +220,109 @@ def fwd_bwd_leg(n: int, state, launches: int = 100) -> dict:
     t_plain = time_launches_us(lambda: render(plain), launches)
     t_fwd = time_launches_us(lambda: render(withvis), launches)
     t_bwd = time_launches_us(backward, launches)
-    # autograd path: diff.render + backward of <gout, img>
-    p = diff.scene_params(r)
+    del keep
+    return {"plain": t_plain, "fwd": t_fwd, "bwd": t_bwd}
 
-    def step():
-        img = diff.render(r, p)
-        img.backward(gout)
+
+def _fwd_bwd_record(r, t: dict, workload: str) -> dict:
+    N, C, H, W = r.output_shape()
+    hw = H * W
+    inst = 0
+    for nd in r._drawn_nodes():
+        if not nd.shared_across:
+            inst += N * nd.instances_per_scene
+    params = 80 * inst + 64 * N
+    b_plain = N * hw * C * 4 + params
+    b_fwd = b_plain + N * hw * 4
+    b_bwd = N * (hw * C * 4 + hw * 4) + params
+    return {
+        "workload": workload,
+        "plain_fwd_us": round(t["plain"], 2),
+        "fwd_with_vis_us": round(t["fwd"], 2),
+        "bwd_us": round(t["bwd"], 2),
+        "fwd_bwd_over_plain_fwd": round((t["fwd"] + t["bwd"]) / t["plain"], 3),
+        "fwd_bwd_scene_frames_per_s": round(N / ((t["fwd"] + t["bwd"]) * 1e-6), 1),
+        "plain_fwd_scene_frames_per_s": round(N / (t["plain"] * 1e-6), 1),
+        "hbm_gbs": {"plain_fwd": round(b_plain / t["plain"] / 1e3, 1),
+                    "fwd_with_vis": round(b_fwd / t["fwd"] / 1e3, 1),
+                    "bwd": round(b_bwd / t["bwd"] / 1e3, 1)},
+        "hbm_frac": {"plain_fwd": round(b_plain / t["plain"] / 1e3 / HBM_PEAK_GBS, 4),
+                     "fwd_with_vis": round(b_fwd / t["fwd"] / 1e3 / HBM_PEAK_GBS, 4),
+                     "bwd": round(b_bwd / t["bwd"] / 1e3 / HBM_PEAK_GBS, 4)},
+    }
+
+
+def _autograd_step_ms(step, iters: int = 20) -> float:
+    import torch
 
     for _ in range(5):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    iters = 20
     for _ in range(iters):
         step()
     torch.cuda.synchronize()
-    t_auto = (time.perf_counter() - t0) / iters
-    del keep
-    hw = H * W
-    b_plain = N * (hw * C * 4 + 224 + 64)
-    b_fwd = b_plain + N * hw * 4
-    b_bwd = N * (hw * C * 4 + hw * 4 + 224 + 64)
-    return {
-        "workload": f"CartPole cart+pole+ground quad, {N} scenes {W}x{H}, float32 RGB",
-        "plain_fwd_us": round(t_plain, 2),
-        "fwd_with_vis_us": round(t_fwd, 2),
-        "bwd_us": round(t_bwd, 2),
-        "fwd_bwd_over_plain_fwd": round((t_fwd + t_bwd) / t_plain, 3),
-        "fwd_bwd_scene_frames_per_s": round(N / ((t_fwd + t_bwd) * 1e-6), 1),
-        "plain_fwd_scene_frames_per_s": round(N / (t_plain * 1e-6), 1),
-        "hbm_gbs": {"plain_fwd": round(b_plain / t_plain / 1e3, 1),
-                    "fwd_with_vis": round(b_fwd / t_fwd / 1e3, 1),
-                    "bwd": round(b_bwd / t_bwd / 1e3, 1)},
-        "autograd_step_ms": round(t_auto * 1e3, 3),
-        "note": "C-ABI launches back to back on one stream, HIP events; autograd_step = "
-                "diff.render + backward incl. parameter copies and gradient zeroing",
-    }
+    return (time.perf_counter() - t0) / iters * 1e3
+
+
+def fwd_bwd_leg(n: int, state, launches: int = 100) -> dict:
+    """Differentiable render of the CartPole workload with float32 output:
+    plain forward (colour only) vs forward with the visibility buffer plus
+    pbr_render_backward (gradients for instance colours, light and vertices),
+    all through the C ABI on one stream; and the same pair through autograd
+    (pybatchrender.diff)."""
+    import torch
+
+    from pybatchrender import diff
+    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=n, device="cuda",
+                                              output_dtype="float32"))
+    r.step(state)
+    N, C, H, W = r.output_shape()
+    gout = torch.randn((N, C, H, W), device="cuda")
+    rec = _fwd_bwd_record(r, _fwd_bwd_launch_times(r, gout, launches),
+                          f"CartPole cart+pole+ground quad, {N} scenes {W}x{H}, float32 RGB")
+    p = diff.scene_params(r)
+
+    def step():
+        diff.render(r, p).backward(gout)
+
+    rec["autograd_step_ms"] = round(_autograd_step_ms(step), 3)
+    rec["note"] = ("C-ABI launches back to back on one stream, HIP events; autograd_step = "
+                   "diff.render + backward incl. parameter copies and gradient zeroing")
+    return rec
+
+
+def differentiable_leg(n: int = 512, launches: int = 50) -> dict:
+    """The differentiable-render workload: 128 x 128 float32, 32 instances
+    per scene (16 boxes, 16 UV spheres), L2 loss to a target rendered from
+    perturbed colours, light direction and vertices.  Plain forward vs
+    forward with visibility + backward of the loss gradient 2 (img -
+    target), C-ABI launches; and one autograd step of the loss."""
+    import torch
+
+    from pybatchrender import diff, workloads
+
+    r, tgt = workloads.differentiable(num_scenes=n)
+    with torch.no_grad():
+        target = diff.render(r, tgt)
+        p = diff.scene_params(r, requires_grad=False)
+        gout = 2.0 * (diff.render(r, p) - target)
+    N, C, H, W = r.output_shape()
+    rec = _fwd_bwd_record(r, _fwd_bwd_launch_times(r, gout, launches),
+                          f"differentiable render, {N} scenes {W}x{H} float32, 16 boxes + 16 "
+                          "UV spheres (16x8) per scene, L2 loss to a perturbed-parameter target")
+    p = diff.scene_params(r)
+
+    def step():
+        for t in p.tensors():
+            t.grad = None
+        ((diff.render(r, p) - target) ** 2).sum().backward()
+
+    rec["autograd_step_ms"] = round(_autograd_step_ms(step, iters=10), 3)
+    rec["kernel"] = render_kernel_name(r)
+    return rec
 
 
 def env_loop_leg(n: int = 16384, steps: int = 1000) -> dict:
@@ -440,6 +506,8 @@ def main() -> int:
                     help="kernel family of the render call (PBRConfig.kernel)")
     ap.add_argument("--bwd-only", action="store_true",
                     help="only the fwd+bwd leg (for kernel profiles of the backward)")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="only the differentiable-render workload's fwd+bwd leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="with --kernel-only: also copy 256 MiB device-to-device 5 times")
     ap.add_argument("--config", default="cartpole",
@@ -488,6 +556,10 @@ def main() -> int:
         print(json.dumps({"kernel_us": kus, "bytes_per_launch": algorithmic_bytes(r)}), flush=True)
         if args.copy_calibration:
             copy_calibration()
+        return 0
+    if args.diff_only:
+        log("[bench] differentiable-render workload only")
+        print(json.dumps(differentiable_leg()), flush=True)
         return 0
     if args.bwd_only:
         log("[bench] fwd+bwd leg only")
@@ -549,6 +621,9 @@ def main() -> int:
     if not args.no_bwd:
         log("[bench] fwd+bwd (differentiable render)")
         rec["fwd_bwd"] = fwd_bwd_leg(n, states[0])
+    if not args.no_bwd:
+        log("[bench] fwd+bwd (differentiable-render workload)")
+        rec["fwd_bwd_differentiable"] = differentiable_leg()
     if not args.no_loop:
         log("[bench] on-device env loop")
         rec["env_loop"] = env_loop_leg(args.loop_scenes, args.loop_steps)

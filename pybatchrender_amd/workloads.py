@@ -9,6 +9,14 @@ uneven_batch   N scenes whose instance count is round(exp(U(0, ln max))) in
                cylinder (32) uniformly, so three nodes carry per-scene counts
                (PBRNode.set_instance_counts).  Transforms/colours/camera as
                many_cubes.
+differentiable N scenes of `instances` instances, half boxes (flat) and half
+               UV spheres (16x8, smooth) at positions U(-1.8, 1.8)^3, scales
+               U(0.5, 1), camera eye (0, -5, 1.8) yawed per scene (about half
+               of each frame covered), float32 output, plus the perturbed
+               parameters a target image is rendered from: colours
+               + N(0, 0.05), light direction rotated by an N(0, 5 deg) angle
+               about a uniformly random axis, model-space vertex positions
+               + N(0, 0.01).
 All random numbers come from torch.Generator('cpu').manual_seed(seed).
 """
 from __future__ import annotations
@@ -48,9 +56,10 @@ def random_transforms(n: int, gen: torch.Generator, pos_range: float = 3.0,
     return M.to(torch.float32)
 
 
-def _yaw_camera(r: PBRRenderer, n: int, gen: torch.Generator, fov: float = 55.0) -> None:
+def _yaw_camera(r: PBRRenderer, n: int, gen: torch.Generator, fov: float = 55.0,
+                eye=(0.0, -8.0, 3.0)) -> None:
     yaw = torch.rand(n, generator=gen, dtype=torch.float64) * 2 * math.pi
-    base = torch.tensor([0.0, -8.0, 3.0], dtype=torch.float64)
+    base = torch.tensor(eye, dtype=torch.float64)
     eye = torch.stack([base[0] * torch.cos(yaw) - base[1] * torch.sin(yaw),
                        base[0] * torch.sin(yaw) + base[1] * torch.cos(yaw),
                        base[2].expand(n)], dim=1).to(torch.float32)
@@ -105,3 +114,52 @@ def uneven_batch(num_scenes: int = 8192, resolution: int = 84, max_instances: in
     r.add_light(**LIGHT)
     r.setup_environment()
     return r
+
+
+def _rotate(v: torch.Tensor, axis: torch.Tensor, angle: float) -> torch.Tensor:
+    """Rodrigues rotation of vector v about the unit axis by angle (radians)."""
+    c, s_ = math.cos(angle), math.sin(angle)
+    return v * c + torch.cross(axis, v, dim=0) * s_ + axis * torch.dot(axis, v) * (1.0 - c)
+
+
+def differentiable(num_scenes: int = 512, resolution: int = 128, instances: int = 32,
+                   seed: int = 0, device: str = "cuda", sphere=(16, 8), **cfg):
+    """Returns (renderer, target): the renderer's own colours, vertices and
+    light are the parameters being fitted; ``target`` is a diff.SceneParams
+    (no gradients) of the perturbed parameters the target image is rendered
+    from (``diff.render(renderer, target)``)."""
+    from . import diff
+    from .renderer.mesh import uv_sphere
+
+    gen = torch.Generator("cpu").manual_seed(seed)
+    c = PBRConfig(num_scenes=num_scenes, tile_resolution=(resolution, resolution), device=device,
+                  output_dtype="float32", **cfg)
+    r = PBRRenderer(c)
+    n_box = instances // 2
+    n_sph = instances - n_box
+    for model, ips in (("models/box", n_box), (uv_sphere(*sphere), n_sph)):
+        if ips == 0:
+            continue
+        node = r.add_node(model, instances_per_scene=ips)
+        rows = num_scenes * ips
+        node.set_transforms(random_transforms(rows, gen, pos_range=1.8, scale_range=(0.5, 1.0)))
+        col = torch.ones(rows, 4)
+        col[:, :3] = torch.rand(rows, 3, generator=gen) * 0.8 + 0.1
+        node.set_colors(col)
+    _yaw_camera(r, num_scenes, gen, eye=(0.0, -5.0, 1.8))
+    # strength below 1: every light entry is differentiable (clamp(strength,
+    # 0, 1) has a kink at 1)
+    r.add_light(**{**LIGHT, "strength": 0.9})
+    r.setup_environment()
+
+    p = diff.scene_params(r, requires_grad=False)
+    cols = [t + (torch.randn(t.shape, generator=gen) * 0.05).to(t.device) for t in p.colors]
+    verts = p.vertices.clone()
+    off = torch.randn(verts.shape[0], 3, generator=gen) * 0.01
+    verts[:, :3] += off.to(verts.device)
+    light = p.light.clone()
+    axis = torch.randn(3, generator=gen, dtype=torch.float64)
+    axis = axis / axis.norm()
+    angle = math.radians(float(torch.randn(1, generator=gen).item()) * 5.0)
+    light[:3] = _rotate(light[:3].double(), axis, angle).to(torch.float32)
+    return r, diff.SceneParams(cols, verts, light)
