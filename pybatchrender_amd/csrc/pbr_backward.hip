@@ -60,7 +60,13 @@ constexpr int kRun = 16;   // pixels of one row walked by a lane
 constexpr int kAcc = 32;   // accumulator floats per slot
 constexpr int kRecF = 24;  // per-pixel factors per slot
 constexpr int kLightF = 10;
-constexpr int kMaxCap = 512;          // slots held in LDS per pass
+constexpr int kMaxCap = 512;          // flat kernel: slots held in LDS
+// per-scene kernel: slots held in LDS per pass (two workgroups fit a CU)
+constexpr int kPassCap = 320;
+// per-scene kernel: rows per band when a scene is split over several
+// workgroups, and the workgroup count up to which scenes are split
+constexpr int kBandRows = 16;
+constexpr int kBandTarget = 4096;
 constexpr int kMaxVtab = 2048;        // vertices whose gradients an LDS table holds
 constexpr int kMaxStab = 1024;        // shared instance rows held in LDS
 constexpr int kMaxBitmapSlots = 1 << 17;
@@ -92,6 +98,8 @@ struct BwdFrame {
   int ttab;        // flat kernel: triangles whose first vertex an LDS table holds
   int P;           // floats per workgroup partial row
   int off_v, off_s;  // partial row: vertex block, shared-colour block
+  int bands;       // per-scene kernel: workgroups (row bands) per scene
+  int band_rows;   // rows of one band
   int soff[PBR_MAX_NODES];  // first shared row of node j, -1 if not shared
 };
 
@@ -340,7 +348,11 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
                                                              const float* gout, float* partial) {
   constexpr int RUN = SMOOTH ? 8 : kRun;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int s = (int)blockIdx.x;
+  // workgroup -> (scene, band of rows [y0, y1))
+  const int s = (int)blockIdx.x / F.bands;
+  const int band = (int)blockIdx.x - s * F.bands;
+  const int y0 = band * F.band_rows;
+  const int y1 = min(F.H, y0 + F.band_rows);
   const int tid = (int)threadIdx.x;
   const int W = F.W, H = F.H, C = F.C;
   const size_t plane = (size_t)W * (size_t)H;
@@ -390,7 +402,7 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
   const uint32_t total = S.base[D.num_nodes];
 
   const int segs = (W + RUN - 1) / RUN;
-  const int nruns = H * segs;
+  const int nruns = max(0, y1 - y0) * segs;
   const int32_t* vscene = vis + (size_t)s * plane;
   const float* gscene = gout + (size_t)s * (size_t)C * plane;
 
@@ -398,7 +410,7 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
   int nvis = (int)total;
   if (F.compact) {
     for (int run = tid; run < nruns; run += kBwdBlock) {
-      const int y = run / segs, x0 = (run - y * segs) * RUN;
+      const int yr = run / segs, x0 = (run - yr * segs) * RUN, y = y0 + yr;
       const int n = min(RUN, W - x0);
       const int32_t* vr = vscene + (size_t)y * W + x0;
       int32_t prev = -1;
@@ -492,7 +504,7 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
 
     // phase 2: pixel runs
     for (int run = tid; run < nruns; run += kBwdBlock) {
-      const int y = run / segs, x0 = (run - y * segs) * RUN;
+      const int yr = run / segs, x0 = (run - yr * segs) * RUN, y = y0 + yr;
       const int n = min(RUN, W - x0);
       const int32_t* vr = vscene + (size_t)y * W + x0;
       const float* gr = gscene + (size_t)y * W + x0;
@@ -726,7 +738,8 @@ __global__ __launch_bounds__(kBwdBlock) void backward_kernel(pbr_scene_desc D, B
   }
 
   // phase 4: this workgroup's partial row
-  write_partial(F, L, S.light, S.vtab, S.stab, partial + (size_t)s * F.P, tid, kBwdBlock);
+  write_partial(F, L, S.light, S.vtab, S.stab, partial + (size_t)blockIdx.x * F.P, tid,
+                kBwdBlock);
 }
 
 // --- flat scenes: persistent, wave-specialised workgroups --------------------
@@ -1165,9 +1178,25 @@ int plan_backward(const pbr_scene_desc* D, int channels, const pbr_grads* g, Bwd
   if (slots > (uint64_t)kMaxBitmapSlots)
     return fail(PBR_ERR_UNSUPPORTED, "backward supports at most %d triangle slots per scene",
                 kMaxBitmapSlots);
-  F.compact = slots > (uint64_t)kMaxCap ? 1 : 0;
-  F.cap = F.compact ? kMaxCap : (int)((slots + 31) / 32 * 32);
+  // the flat kernel holds every slot of a scene (at most kMaxCap); the
+  // per-scene kernel numbers the visible slots of its band when there are
+  // more than kPassCap and takes them in passes of kPassCap
+  P.flat = !any_smooth(D) && slots <= (uint64_t)kMaxCap;
+  const int capmax = P.flat ? kMaxCap : kPassCap;
+  F.compact = slots > (uint64_t)capmax ? 1 : 0;
+  F.cap = F.compact ? capmax : (int)((slots + 31) / 32 * 32);
   if (F.cap == 0) F.cap = 32;
+  F.band_rows = F.H;
+  F.bands = 1;
+  if (!P.flat && D->num_scenes > 0 && D->num_scenes < kBandTarget) {
+    // split scenes into row bands while the grid is small: more workgroups
+    // in flight, and fewer visible slots per workgroup (fewer passes)
+    const int want = (kBandTarget + D->num_scenes - 1) / D->num_scenes;
+    const int most = (F.H + kBandRows - 1) / kBandRows;
+    const int b = std::max(1, std::min(want, most));
+    F.band_rows = (F.H + b - 1) / b;
+    F.bands = (F.H + F.band_rows - 1) / F.band_rows;
+  }
   F.nwords = F.compact ? (int)((slots + 31) / 32) : 0;
   int nverts = 0;
   for (int m = 0; m < D->num_meshes; ++m)
@@ -1187,7 +1216,6 @@ int plan_backward(const pbr_scene_desc* D, int channels, const pbr_grads* g, Bwd
   F.off_v = kLightF;
   F.off_s = F.off_v + F.vtab * 6;
   F.P = F.off_s + F.stab * 4;
-  P.flat = !any_smooth(D) && !F.compact;
   if (P.flat) {
     int ntri = 0;
     for (int m = 0; m < D->num_meshes; ++m)
@@ -1205,8 +1233,9 @@ int plan_backward(const pbr_scene_desc* D, int channels, const pbr_grads* g, Bwd
             sizeof(float) * 16 + sizeof(float) * (size_t)F.vtab * 6 +
             sizeof(float) * (size_t)F.stab * 4 + sizeof(uint32_t) * (PBR_MAX_NODES + 1) +
             sizeof(uint32_t) * (kBwdBlock + 1);
-  // one partial row per workgroup: per scene, or per persistent flat workgroup
-  P.rows = P.flat ? min(D->num_scenes, kMaxFlatGrid) : D->num_scenes;
+  // one partial row per workgroup: per (scene, band), or per persistent flat
+  // workgroup
+  P.rows = P.flat ? min(D->num_scenes, kMaxFlatGrid) : D->num_scenes * F.bands;
   P.row_bytes = sizeof(float) * (size_t)F.P;
   P.partial_bytes = (P.row_bytes * (size_t)P.rows + 255) / 256 * 256;
   if (P.lds > 160 * 1024)
@@ -1358,15 +1387,17 @@ int pbr_render_backward(const pbr_scene_desc* scene, const int32_t* vis, const f
     if (e != hipSuccess)
       return fail(PBR_ERR_LAUNCH, "backward LDS request failed: %s", hipGetErrorString(e));
   }
+  const uint64_t wgs = (uint64_t)scene->num_scenes * (uint64_t)P.F.bands;
+  if (wgs >= 0x7fffffffull) return fail(PBR_ERR_INVALID, "too many backward workgroups");
   if (smooth)
-    hipLaunchKernelGGL(backward_kernel<true>, dim3(scene->num_scenes), dim3(kBwdBlock), P.lds, st,
+    hipLaunchKernelGGL(backward_kernel<true>, dim3((uint32_t)wgs), dim3(kBwdBlock), P.lds, st,
                        *scene, P.F, L, G, vis, grad_color, partial);
   else
-    hipLaunchKernelGGL(backward_kernel<false>, dim3(scene->num_scenes), dim3(kBwdBlock), P.lds, st,
+    hipLaunchKernelGGL(backward_kernel<false>, dim3((uint32_t)wgs), dim3(kBwdBlock), P.lds, st,
                        *scene, P.F, L, G, vis, grad_color, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "backward launch failed: %s", hipGetErrorString(e));
-  return launch_reduce(scene, P, G, scene->num_scenes, partial, st);
+  return launch_reduce(scene, P, G, P.rows, partial, st);
 }
 
 }  // extern "C"
