@@ -373,15 +373,8 @@ def env_loop_leg(n: int = 16384, steps: int = 1000) -> dict:
 
 
 def render_kernel_name(r) -> str:
-    """The kernel family the library's AUTO choice runs for this renderer
-    (the rule of choose_kernel in csrc/pbr_render.hip)."""
-    if r.cfg.kernel != "auto":
-        return r.cfg.kernel
-    slots = sum(n.instances_per_scene * n.mesh.num_triangles for n in r._drawn_nodes())
-    W = int(r.cfg.tile_resolution[0])
-    if not r.cfg.persistent and slots <= 64 and W <= 128:
-        return "rows"
-    return "scene" if not r.cfg.persistent and slots <= 64 else "tiles"
+    """The kernel family pbr_render runs for this renderer (pbr_choose_kernel)."""
+    return r._chosen_kernel()
 
 
 KERNEL_LABEL = {"rows": "rows_kernel (setup + lists + row walk, one launch)",

@@ -252,6 +252,14 @@ int pbr_cartpole_step(const pbr_cartpole_params* params, const pbr_cartpole_batc
 int pbr_cartpole_reset(const pbr_cartpole_params* params, const pbr_cartpole_batch* batch,
                        void* stream);
 
+/* The kernel family pbr_render runs for this batch and launch shape: one
+ * of PBR_KERNEL_CHUNKED / _SCENE / _TILES / _ROWS (AUTO resolved) in
+ * *kernel.  No counterpart in the reference; a caller reads the tile path's
+ * overflow flag (the first uint32 of its workspace: nonzero when per-scene
+ * instance counts exceeded opts.slot_budget) only after a _TILES render. */
+int pbr_choose_kernel(const pbr_scene_desc* scene, const pbr_render_opts* opts,
+                      int32_t* kernel);
+
 /* Device workspace pbr_render needs for this batch and launch shape.
  * No counterpart in the reference (GL owned its buffers). */
 int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts,

@@ -213,6 +213,7 @@ EXPORTS = (
     "pbr_cartpole_step",
     "pbr_cartpole_reset",
     "pbr_workspace_bytes",
+    "pbr_choose_kernel",
     "pbr_last_error",
     "pbr_abi_version",
     "pbr_abi_layout",
@@ -271,6 +272,12 @@ def declare(lib: C.CDLL) -> C.CDLL:
         C.POINTER(C.c_size_t),
     ]
     lib.pbr_workspace_bytes.restype = C.c_int
+    lib.pbr_choose_kernel.argtypes = [
+        C.POINTER(pbr_scene_desc),
+        C.POINTER(pbr_render_opts),
+        C.POINTER(C.c_int32),
+    ]
+    lib.pbr_choose_kernel.restype = C.c_int
     lib.pbr_last_error.argtypes = []
     lib.pbr_last_error.restype = C.c_char_p
     lib.pbr_abi_version.argtypes = []

@@ -63,6 +63,15 @@ def workspace_bytes(desc: _abi.pbr_scene_desc, opts: _abi.pbr_render_opts) -> in
     return int(n.value)
 
 
+def choose_kernel(desc: _abi.pbr_scene_desc, opts: _abi.pbr_render_opts) -> int:
+    """PBR_KERNEL_* that pbr_render runs for this descriptor and options."""
+    k = ctypes.c_int32(0)
+    rc = library().pbr_choose_kernel(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(k))
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_choose_kernel failed (status {rc}): {last_error()}")
+    return int(k.value)
+
+
 def set_pose(desc: _abi.pbr_pose_desc, stream_ptr: int) -> None:
     rc = library().pbr_set_pose(ctypes.byref(desc), ctypes.c_void_p(stream_ptr or None))
     if rc != _abi.PBR_OK:

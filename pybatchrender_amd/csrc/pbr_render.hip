@@ -1199,6 +1199,16 @@ int pbr_set_pose(const pbr_pose_desc* pose, void* stream) {
   return pbr_set_poses(pose, 1, stream);
 }
 
+int pbr_choose_kernel(const pbr_scene_desc* scene, const pbr_render_opts* opts,
+                      int32_t* kernel) {
+  if (!kernel) return fail(PBR_ERR_INVALID, "null kernel");
+  const int rc = validate_scene(scene);
+  if (rc) return rc;
+  g_err[0] = 0;
+  *kernel = choose_kernel(scene, opts);
+  return PBR_OK;
+}
+
 int pbr_workspace_bytes(const pbr_scene_desc* scene, const pbr_render_opts* opts, size_t* bytes) {
   if (!bytes) return fail(PBR_ERR_INVALID, "null bytes");
   if (!scene) return fail(PBR_ERR_INVALID, "null descriptor");

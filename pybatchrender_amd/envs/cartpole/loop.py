@@ -99,9 +99,15 @@ class CartPoleLoop:
         if depth is None:
             depth = bool(renderer.cfg.output_depth)
         self.depth = torch.empty((N, H, W), dtype=torch.float32, device=dev) if depth else None
+        # the loop's own workspace (the graph keeps using it whatever the
+        # renderer later renders eagerly) and the descriptor's tensors, held
+        # for as long as the loop may replay its graph
+        self.workspace = torch.empty(max(renderer.workspace_bytes(), 256), dtype=torch.uint8,
+                                     device=dev)
         self.graph = None
         if graph:
             self._capture()
+        self._keep = list(renderer._desc_cache[2])
         # episodes (and the random streams) start at the reset
         self.state.zero_()
         self.step_count.zero_()
@@ -114,7 +120,8 @@ class CartPoleLoop:
 
     def _body(self) -> None:
         _native.cartpole_step(self.params, self.batch, self.poses, self._stream())
-        self.r.render(out=self.pixels, depth=self.depth if self.depth is not None else False)
+        self.r.render(out=self.pixels, depth=self.depth if self.depth is not None else False,
+                      workspace=self.workspace)
 
     def _capture(self, warmup: int = 2) -> None:
         dev = self.r.device
@@ -132,7 +139,8 @@ class CartPoleLoop:
         """Fresh states for all scenes (env.py:153-176) and their frame."""
         _native.cartpole_reset(self.params, self.batch, self._stream())
         self.r._step(self.state)
-        self.r.render(out=self.pixels, depth=self.depth if self.depth is not None else False)
+        self.r.render(out=self.pixels, depth=self.depth if self.depth is not None else False,
+                      workspace=self.workspace)
         self.done.zero_()
         return self.outputs()
 

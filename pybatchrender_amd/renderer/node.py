@@ -72,6 +72,7 @@ class PBRNode(PBRShaderContext):
         self.name = name or (model_path if isinstance(model_path, str) else "mesh")
         self.instance_counts: torch.Tensor | None = None
         self.instance_count_total = 0
+        self._counts_version = 0
 
         if model_path is not None and (not isinstance(model_path, str) or model_path):
             raw = model_path if isinstance(model_path, Mesh) else load_model(model_path)
@@ -295,6 +296,19 @@ class PBRNode(PBRShaderContext):
         self.instance_counts = c.clamp(0, self.instances_per_scene).contiguous()
         # the batch's instance total sizes the tile path's record pool (one sync here)
         self.instance_count_total = int(self.instance_counts.sum().item())
+        self._counts_version = self.instance_counts._version
+
+    def counts_total(self) -> int:
+        """Sum of the per-scene instance counts (each clamped to [0, ips]),
+        recomputed with one device sync if the counts tensor was written in
+        place since the last time."""
+        c = self.instance_counts
+        if c is None:
+            return self.num_scenes * self.instances_per_scene
+        if c._version != self._counts_version:
+            self.instance_count_total = int(c.clamp(0, self.instances_per_scene).sum().item())
+            self._counts_version = c._version
+        return self.instance_count_total
 
     # --- misc API of the reference ----------------------------------------
     def set_texture(self, texture=None) -> None:

@@ -166,6 +166,10 @@ class PBRRenderer:
         for n in g["nodes"]:
             key += [n.transforms_b44, n.colors_b4, n.instance_counts, n.instances_per_scene,
                     n.shared_across]
+            if n.instance_counts is not None:
+                # in-place edits of the counts change the tile path's slot
+                # budget: torch's version counter of the tensor notices them
+                key.append(n.instance_counts._version)
         return key
 
     @staticmethod
@@ -254,9 +258,12 @@ class PBRRenderer:
         return (self.num_scenes, int(self.cfg.num_channels), int(H), int(W))
 
     def render(self, out: torch.Tensor | None = None, depth: torch.Tensor | bool | None = None,
-               vis: torch.Tensor | bool | None = None) -> dict:
+               vis: torch.Tensor | bool | None = None,
+               workspace: torch.Tensor | None = None) -> dict:
         """Render the current scene state.  Returns {'pixels', 'depth'?, 'vis'?};
-        pass preallocated tensors to render in place (graph capture)."""
+        pass preallocated tensors to render in place (graph capture).
+        ``workspace`` (uint8, on the device) replaces the renderer's own
+        workspace for this call; it must hold ``workspace_bytes()``."""
         if self.device.type != "cuda":
             raise RuntimeError(
                 f"the HIP renderer needs a GPU device (cfg.device='{self.cfg.device}')")
@@ -293,7 +300,13 @@ class PBRRenderer:
             ws_bytes = _native.workspace_bytes(desc, opts)
             self._ws_cache = (self._desc_cache, opts, ws_bytes)
         ws_ptr = 0
-        if ws_bytes:
+        if workspace is not None:
+            if (workspace.dtype != torch.uint8 or workspace.device != self.device
+                    or not workspace.is_contiguous() or workspace.numel() < ws_bytes):
+                raise ValueError(f"workspace must be a contiguous uint8 tensor of at least "
+                                 f"{ws_bytes} bytes on {self.device}")
+            ws_ptr = workspace.data_ptr() if ws_bytes else 0
+        elif ws_bytes:
             if self._workspace is None or self._workspace.numel() < ws_bytes:
                 self._workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
             ws_ptr = self._workspace.data_ptr()
@@ -340,16 +353,46 @@ class PBRRenderer:
 
     def slot_budget(self) -> int:
         """Triangle slots of the whole batch when some node draws per-scene
-        instance counts (sizes the tile path's record pool); 0 otherwise."""
+        instance counts (sizes the tile path's record pool); 0 otherwise.
+        A node's count total is recomputed (one device sync) when its counts
+        tensor was written in place since the total was taken."""
         nodes = self._drawn_nodes()
         if not any(n.instance_counts is not None for n in nodes):
             return 0
         total = 0
         for n in nodes:
-            inst = (n.instance_count_total if n.instance_counts is not None
-                    else self.num_scenes * n.instances_per_scene)
+            if n.instance_counts is not None:
+                inst = n.counts_total()
+            else:
+                inst = self.num_scenes * n.instances_per_scene
             total += inst * n.mesh.num_triangles
         return int(total)
+
+    def workspace_bytes(self) -> int:
+        """Device workspace the next render of the current state needs."""
+        desc, keep = self.scene_descriptor()
+        n = _native.workspace_bytes(desc, self._cached_opts())
+        del keep
+        return n
+
+    def last_render_overflowed(self, workspace: torch.Tensor | None = None) -> bool:
+        """True if the last tile-path render with per-scene instance counts
+        found more triangle slots than its record pool holds (the slots past
+        the pool were not drawn).  Synchronises the device.  Counts written
+        in place through torch update the pool size by themselves; this flag
+        catches counts a kernel wrote behind torch's back.  False for the
+        kernels that keep no record pool."""
+        ws = workspace if workspace is not None else self._workspace
+        if ws is None or self.slot_budget() == 0 or self._chosen_kernel() != "tiles":
+            return False
+        return bool(ws[:4].view(torch.int32)[0].item() != 0)
+
+    def _chosen_kernel(self) -> str:
+        desc, keep = self.scene_descriptor()
+        code = _native.choose_kernel(desc, self._cached_opts())
+        del keep
+        return {_abi.PBR_KERNEL_CHUNKED: "chunked", _abi.PBR_KERNEL_SCENE: "scene",
+                _abi.PBR_KERNEL_TILES: "tiles", _abi.PBR_KERNEL_ROWS: "rows"}[code]
 
     def set_poses(self, poses) -> None:
         """PBRNode.set_pose for several nodes, one HIP launch on the GPU.
@@ -415,12 +458,21 @@ class PBRRenderer:
 class GraphedStep:
     """PBRRenderer.step as one captured hipGraph.
 
-    ``g(state)`` copies ``state`` into the graph's static input, replays the
-    graph (the renderer's ``_step`` launches followed by the render launch)
-    and returns {'pixels', 'depth'?, 'vis'?}.  The returned tensors are the
-    graph's static outputs: the next call overwrites them, clone to keep.
-    Scene changes outside ``_step`` (new nodes, a replaced transforms tensor)
-    need a new capture; values written in place are picked up."""
+    ``g(state)`` copies ``state`` into the graph's static input (no copy when
+    ``state`` is that input, ``g.state``), replays the graph (the renderer's
+    ``_step`` launches followed by the render launch) and returns {'pixels',
+    'depth'?, 'vis'?}.  The returned tensors are the graph's static outputs:
+    the next call overwrites them, clone to keep.
+
+    The graph renders the scene as it was at capture: scene changes outside
+    ``_step`` (new nodes, a replaced transforms tensor) need a new capture;
+    values written in place are picked up.  The graph owns its workspace
+    and keeps every tensor it reads alive, so eager renders, descriptor
+    rebuilds or ``destroy()`` of the renderer do not touch its memory.
+    Per-scene instance counts may be edited in place between replays as
+    long as the batch total stays within the total at capture (the record
+    pool of the tile path is sized then): a call after such an edit checks
+    it (one device sync) and raises if it grew beyond."""
 
     def __init__(self, renderer: PBRRenderer, example_state: torch.Tensor | None = None,
                  depth: bool = False, vis: bool = False, warmup: int = 2):
@@ -436,6 +488,10 @@ class GraphedStep:
         self.out = torch.empty((N, C, H, W), dtype=tdtype, device=dev)
         self.depth = torch.empty((N, H, W), dtype=torch.float32, device=dev) if depth else None
         self.vis = torch.empty((N, H, W), dtype=torch.int32, device=dev) if vis else None
+        if self.state is not None:
+            renderer._step(self.state)
+        self.workspace = torch.empty(max(renderer.workspace_bytes(), 256), dtype=torch.uint8,
+                                     device=dev)
         side = torch.cuda.Stream(dev)
         side.wait_stream(torch.cuda.current_stream(dev))
         with torch.cuda.stream(side):
@@ -445,12 +501,19 @@ class GraphedStep:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self._body()
+        # what the graph reads: the descriptor's tensors (geometry, node
+        # buffers, view-projection, counts); held so they outlive renderer
+        # changes
+        self._keep = list(renderer._desc_cache[2])
+        self._budget = renderer.slot_budget()
+        self._counts = [(n, n._counts_version) for n in renderer._drawn_nodes()
+                        if n.instance_counts is not None]
 
     def _body(self) -> None:
         if self.state is not None:
             self.r._step(self.state)
         self.r.render(out=self.out, depth=self.depth if self.depth is not None else False,
-                      vis=self.vis if self.vis is not None else False)
+                      vis=self.vis if self.vis is not None else False, workspace=self.workspace)
 
     def outputs(self) -> dict:
         res = {"pixels": self.out}
@@ -460,7 +523,23 @@ class GraphedStep:
             res["vis"] = self.vis
         return res
 
+    def _check_counts(self) -> None:
+        if not any(n.instance_counts._version != v for n, v in self._counts):
+            return
+        if self.r.slot_budget() > self._budget:
+            raise RuntimeError(
+                "instance counts written in place since the capture exceed the captured batch "
+                f"total ({self._budget} triangle slots): capture the step again")
+        self._counts = [(n, n._counts_version) for n, _ in self._counts]
+
+    def overflowed(self) -> bool:
+        """True if the last replay's tile-path record pool was too small for
+        its instance counts (see PBRRenderer.last_render_overflowed)."""
+        return self.r.last_render_overflowed(self.workspace)
+
     def __call__(self, state: torch.Tensor | None = None) -> dict:
+        if self._counts:
+            self._check_counts()
         if state is not None:
             if self.state is None:
                 raise ValueError("captured without a state input")

@@ -127,3 +127,40 @@ def test_rows_list_cap_is_validated():
     with pytest.raises(ValueError):
         pbr.PBRConfig(num_scenes=1, device="cpu", row_list_cap=512)
     assert pbr.PBRConfig(num_scenes=1, device="cpu", row_list_cap=8).row_list_cap == 8
+
+
+def test_kernel_choice_without_gpu():
+    """pbr_choose_kernel resolves AUTO on the host (no GPU call)."""
+    from pybatchrender.envs.cartpole import CartPoleConfig, CartPoleGroundRenderer
+    from pybatchrender import workloads
+
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=8, device="cpu"))
+    assert r._chosen_kernel() == "rows"
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=8, device="cpu", tile_resolution=(130, 64)))
+    assert r._chosen_kernel() == "scene"
+    r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=8, device="cpu", persistent=True))
+    assert r._chosen_kernel() == "tiles"
+    assert workloads.many_cubes(num_scenes=4, device="cpu")._chosen_kernel() == "tiles"
+    r = workloads.many_cubes(num_scenes=4, device="cpu", kernel="chunked")
+    assert r._chosen_kernel() == "chunked"
+
+
+def test_slot_budget_follows_in_place_count_edits():
+    """Counts written in place through torch change the tile path's pool size
+    (the budget is re-derived from the counts, not cached past the edit)."""
+    from pybatchrender import workloads
+
+    r = workloads.uneven_batch(num_scenes=16, resolution=32, max_instances=8, seed=1, device="cpu")
+    nodes = r._drawn_nodes()
+    b0 = r.slot_budget()
+    before = [int(n.instance_counts.sum()) for n in nodes]
+    nodes[0].instance_counts.fill_(nodes[0].instances_per_scene)
+    b1 = r.slot_budget()
+    assert b1 - b0 == (16 * nodes[0].instances_per_scene - before[0]) * nodes[0].mesh.num_triangles
+    r.scene_descriptor()
+    assert r._cached_opts().slot_budget == b1
+    nodes[1].instance_counts[:4] = 0
+    b2 = r.slot_budget()
+    assert b2 < b1
+    r.scene_descriptor()
+    assert r._cached_opts().slot_budget == b2
