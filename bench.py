@@ -458,7 +458,7 @@ def cpu_baseline(r, sample_scenes: int = 256, min_seconds: float = 8.0,
             "sample": f"{N} scenes of the same workload (first {N} states); median of "
                       f"{len(t_mt)} renders on {threads} threads ({t_mt.sum():.1f} s) after 3 "
                       f"warm-up renders; single_thread: median of {len(t_1)} renders on 1 "
-                      f"thread; C oracle"}
+                      f"thread; C oracle built -O3 -march=x86-64-v3 -ffp-contract=off"}
 
 
 def env_step_rate(n: int, steps: int) -> float:
