@@ -6,6 +6,13 @@ obstacle loop of _check_collisions (:239-287) literally: while the next
 obstacle's y is at or behind the player, interpolate the player's x at that
 y between the previous and current positions, count a hit within the
 collision threshold, advance.  Float64 arithmetic, one scene at a time.
+
+Pinned by closed forms (tests/test_steering_pins.py): passages =
+floor((y - d0) / spacing) + 1 when driving straight, speed + increment per
+passage, crashes exactly one grace period apart on an obstacle line, the
+hit decided at the obstacle's y (a diagonal move through the obstacle hits
+with both end points outside the threshold and misses with its end point
+inside it), x clamped to the road and speed to its minimum.
 """
 from __future__ import annotations
 
