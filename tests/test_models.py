@@ -218,3 +218,42 @@ def test_egg_round_trip_of_a_builtin(tmp_path):
     assert m.num_triangles == src.num_triangles
     np.testing.assert_allclose(m.positions[m.triangles], src.positions[src.triangles])
     np.testing.assert_allclose(m.normals[m.triangles], src.normals[src.triangles])
+
+
+# --- model_hpr: Panda3D's setHpr convention (reference node.py:262-268) -------------
+# Heading turns about +z, pitch about +x, roll about +y (right-handed, degrees),
+# applied roll first, then pitch, then heading.  Unit vectors under quarter
+# turns land on the axes below; compositions show the order.
+@pytest.mark.parametrize("hpr,v,want", [
+    ((90, 0, 0), (0, 1, 0), (-1, 0, 0)),   # heading 90: forward (+y) turns to -x
+    ((90, 0, 0), (1, 0, 0), (0, 1, 0)),
+    ((0, 90, 0), (0, 1, 0), (0, 0, 1)),    # pitch 90: forward points up
+    ((0, 90, 0), (0, 0, 1), (0, -1, 0)),
+    ((0, 0, 90), (1, 0, 0), (0, 0, -1)),   # roll 90 about forward: right side down
+    ((0, 0, 90), (0, 0, 1), (1, 0, 0)),
+    ((90, 90, 0), (0, 1, 0), (0, 0, 1)),   # pitch, then heading
+    ((90, 90, 0), (1, 0, 0), (0, 1, 0)),
+    ((0, 90, 90), (1, 0, 0), (0, 1, 0)),   # roll, then pitch
+    ((90, 0, 90), (0, 0, 1), (0, 1, 0)),   # roll, then heading
+    ((180, 0, 0), (1, 2, 3), (-1, -2, 3)),
+    ((0, 0, 0), (1, 2, 3), (1, 2, 3)),
+])
+def test_model_hpr_quarter_turns(hpr, v, want):
+    from pybatchrender.renderer.mesh import Mesh, bake
+
+    P = np.array([v, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)], dtype=np.float64)
+    N = np.array([v, v, v], dtype=np.float64) / np.linalg.norm(v)
+    m = bake(Mesh(P, N, np.array([[0, 1, 2]]), "probe"), model_hpr=hpr)
+    np.testing.assert_allclose(m.positions[0], want, atol=1e-12)
+    # normals turn with the mesh (a rotation is its own inverse transpose)
+    np.testing.assert_allclose(m.normals[0], np.array(want) / np.linalg.norm(want), atol=1e-12)
+
+
+def test_model_hpr_then_scale_order():
+    """Scale is applied before the rotation (node.py:224-268): a box scaled
+    along x and then turned by heading 90 is long along y."""
+    from pybatchrender.renderer.mesh import bake, box
+
+    m = bake(box(), model_scale=(4.0, 1.0, 1.0), model_hpr=(90, 0, 0))
+    ext = m.positions.max(0) - m.positions.min(0)
+    np.testing.assert_allclose(ext, [1.0, 4.0, 1.0], atol=1e-12)
