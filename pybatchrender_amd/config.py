@@ -93,85 +93,42 @@ class PBRConfig:
     row_list_cap: int = 0
 
     def process_resolution(self) -> None:
-        """Atlas tiling rules of config.py:61-126 (cols = ceil(sqrt(N)))."""
+        """Atlas layout (reference config.py:61-126).  An int ``tiles`` (or
+        ``num_scenes`` when ``tiles`` is None) becomes a grid of ceil(sqrt(n))
+        columns; then ``window_resolution = tiles x tile_resolution`` fills in
+        whichever of the three is missing (64 x 64 tiles and a 1 x 1 grid
+        when two or more are).  ``num_scenes`` must fit the grid and
+        ``batch_inner_dim`` equals its size."""
         if self.tiles is None and self.num_scenes is not None:
             self.tiles = self.num_scenes
-
         if isinstance(self.tiles, int):
-            cols = math.ceil(math.sqrt(self.tiles))
-            rows = math.ceil(self.tiles / cols)
-            self.tiles = (cols, rows)
-
-        num_nones = sum(
-            1 for x in [self.tiles, self.tile_resolution, self.window_resolution] if x is None
-        )
-
-        if num_nones == 0:
-            if (
-                self.window_resolution[0] != self.tiles[0] * self.tile_resolution[0]
-                or self.window_resolution[1] != self.tiles[1] * self.tile_resolution[1]
-            ):
-                raise ValueError(
-                    "window_resolution must equal tiles * tile_resolution. "
-                    f"Got window_resolution={self.window_resolution}, tiles={self.tiles}, "
-                    f"tile_resolution={self.tile_resolution}."
-                )
-        elif num_nones > 1:
-            default_tiles = (1, 1)
-            default_tile_resolution = (64, 64)
-            if self.tiles is not None:
-                self.tile_resolution = default_tile_resolution
-                self.window_resolution = (64 * self.tiles[0], 64 * self.tiles[1])
-            elif self.tile_resolution is not None:
-                self.tiles = default_tiles
-                self.window_resolution = (self.tile_resolution[0], self.tile_resolution[1])
-            elif self.window_resolution is not None:
-                self.tiles = default_tiles
-                self.tile_resolution = (self.window_resolution[0], self.window_resolution[1])
-            else:
-                self.tiles = default_tiles
-                self.tile_resolution = default_tile_resolution
-                self.window_resolution = (
-                    default_tile_resolution[0] * default_tiles[0],
-                    default_tile_resolution[1] * default_tiles[1],
-                )
-        else:
-            if self.tiles is None:
-                self.tiles = (
-                    self.window_resolution[0] // self.tile_resolution[0],
-                    self.window_resolution[1] // self.tile_resolution[1],
-                )
-            elif self.tile_resolution is None:
-                self.tile_resolution = (
-                    self.window_resolution[0] // self.tiles[0],
-                    self.window_resolution[1] // self.tiles[1],
-                )
-            elif self.window_resolution is None:
-                self.window_resolution = (
-                    self.tiles[0] * self.tile_resolution[0],
-                    self.tiles[1] * self.tile_resolution[1],
-                )
-
+            self.tiles = _grid_for(self.tiles)
+        self.tiles, self.tile_resolution, self.window_resolution = _layout(
+            self.tiles, self.tile_resolution, self.window_resolution)
+        cells = self.tiles[0] * self.tiles[1]
         if self.num_scenes is None:
-            self.num_scenes = self.tiles[0] * self.tiles[1]
-        elif self.num_scenes > self.tiles[0] * self.tiles[1]:
-            raise ValueError(f"{self.num_scenes=} can't fit into {self.tiles=}")
-
+            self.num_scenes = cells
+        elif self.num_scenes > cells:
+            raise ValueError(f"{self.num_scenes} scenes do not fit a {self.tiles[0]} x "
+                             f"{self.tiles[1]} atlas")
         if self.batch_inner_dim is None:
-            self.batch_inner_dim = self.tiles[0] * self.tiles[1]
-        elif self.batch_inner_dim != self.tiles[0] * self.tiles[1]:
-            raise ValueError("batch_inner_dim must equal tiles[0] * tiles[1]")
+            self.batch_inner_dim = cells
+        elif self.batch_inner_dim != cells:
+            raise ValueError(f"batch_inner_dim {self.batch_inner_dim} differs from the atlas "
+                             f"size {cells}")
 
     def process_device(self) -> None:
-        """Device rules of config.py:128-145 ('cuda' is the ROCm GPU)."""
-        if self.device is not None and self.device not in ["cpu", "cuda", "mps"]:
-            raise ValueError(f"Invalid device: {self.device}")
+        """Device rules (reference config.py:128-145): 'cpu', 'cuda' (the
+        ROCm GPU here) or 'mps'; None picks the GPU when one is present."""
         if self.device is None:
             self.device = "cuda" if torch.cuda.is_available() else "cpu"
-        if self.device == "cuda" and not torch.cuda.is_available():
-            raise RuntimeError("device is set to CUDA but CUDA is not available")
-        elif self.device == "mps" and not torch.backends.mps.is_available():
-            raise RuntimeError("device is set to MPS but MPS is not available")
+            return
+        present = {"cpu": lambda: True, "cuda": torch.cuda.is_available,
+                   "mps": torch.backends.mps.is_available}
+        if self.device not in present:
+            raise ValueError(f"device must be one of {sorted(present)}, not {self.device!r}")
+        if not present[self.device]():
+            raise RuntimeError(f"device {self.device!r} was asked for but is not available")
 
     def process_output(self) -> None:
         if self.output_dtype not in ("uint8", "bfloat16", "float32"):
@@ -198,32 +155,43 @@ class PBRConfig:
 
     @classmethod
     def from_config(cls: type[T], cfg: T | dict | None = None, **overrides) -> T:
-        """Merge rules of config.py:152-174."""
+        """A config from another one of this class (its own class is kept,
+        so subclasses survive), from a dict of fields, or from defaults;
+        keyword overrides win (reference config.py:152-174)."""
         if isinstance(cfg, cls):
-            cfg_dict = asdict(cfg)
-            cls = cfg.__class__
-        elif isinstance(cfg, dict):
-            cfg_dict = dict(cfg)
+            target, fields = type(cfg), asdict(cfg)
         else:
-            cfg_dict = {}
-        if overrides:
-            cfg_dict.update(overrides)
-        return cls(**cfg_dict)
+            target, fields = cls, (dict(cfg) if isinstance(cfg, dict) else {})
+        return target(**{**fields, **overrides})
 
-    def build_prc(self) -> str:
-        """PRC text the reference fed Panda3D (config.py:176-191); informational."""
-        if self.window_resolution is None:
-            self.process_resolution()
-        prc = [
-            f"window-type {'offscreen' if self.offscreen else 'onscreen'}\n",
-            f"win-size {self.window_resolution[0]} {self.window_resolution[1]}\n",
-        ]
-        prc.append("gl-version 3 2\n" if self.panda3d_backend == "arm" else "threading-model Cull/Draw\n")
-        prc.append(self.extra_prc_file_data)
-        return "".join(prc)
 
-    def __repr__(self):
-        s = f"{type(self).__name__}(\n"
-        for key, value in asdict(self).items():
-            s += f"    {key}: {value!r}\n"
-        return s + ")"
+_DEFAULT_TILE = (64, 64)
+
+
+def _grid_for(n: int) -> tuple[int, int]:
+    """Columns and rows of the smallest near-square grid holding n cells."""
+    cols = math.ceil(math.sqrt(n))
+    return cols, math.ceil(n / cols)
+
+
+def _layout(tiles, tile, window):
+    """(tiles, tile_resolution, window_resolution) with window = tiles x tile,
+    filling in what is missing."""
+    given = [v is not None for v in (tiles, tile, window)]
+    mul = lambda a, b: (a[0] * b[0], a[1] * b[1])  # noqa: E731
+    div = lambda a, b: (a[0] // b[0], a[1] // b[1])  # noqa: E731
+    if all(given):
+        if tuple(window) != mul(tiles, tile):
+            raise ValueError(f"window_resolution {tuple(window)} is not tiles {tuple(tiles)} "
+                             f"times tile_resolution {tuple(tile)}")
+        return tiles, tile, window
+    if sum(given) == 2:
+        if window is None:
+            return tiles, tile, mul(tiles, tile)
+        if tile is None:
+            return tiles, div(window, tiles), window
+        return div(window, tile), tile, window
+    if tiles is not None:
+        return tiles, _DEFAULT_TILE, mul(tiles, _DEFAULT_TILE)
+    one = tuple(tile if tile is not None else (window if window is not None else _DEFAULT_TILE))
+    return (1, 1), one, one

@@ -279,3 +279,62 @@ def test_autotune_candidates_and_signature():
     autotune._apply(r, {"kernel": "tiles", "tile_size": 32, "persistent": False})
     o = r.render_opts()
     assert o.kernel == 3 and o.tile_size == 32 and o.persistent == 0
+
+
+def test_camera_setters_closed_forms():
+    """The basis setters (camera.py:268-344): right x forward gives the up
+    hint; a yaw of 90 deg about +z turns the default forward +y to -x."""
+    r = PBRRenderer(PBRConfig(num_scenes=2, device="cpu"))
+    cam = r.add_camera()
+    assert torch.allclose(cam.get_forward(), torch.tensor([0.0, 1.0, 0.0]).expand(2, 3))
+    assert torch.allclose(cam.get_up(), torch.tensor([0.0, 0.0, 1.0]).expand(2, 3))
+    assert torch.allclose(cam.get_right(), torch.tensor([1.0, 0.0, 0.0]).expand(2, 3))
+    cam.set_right(torch.tensor([0.0, 0.0, -1.0]))  # roll the camera: up becomes +x
+    assert torch.allclose(cam.get_right(), torch.tensor([0.0, 0.0, -1.0]).expand(2, 3), atol=1e-6)
+    assert torch.allclose(cam.get_up(), torch.tensor([1.0, 0.0, 0.0]).expand(2, 3), atol=1e-6)
+    cam.set_hprs(torch.tensor([[math.pi / 2, 0.0, 0.0], [0.0, 0.0, 0.0]]))
+    assert torch.allclose(cam.get_forward()[0], torch.tensor([-1.0, 0.0, 0.0]), atol=1e-6)
+    assert torch.allclose(cam.get_forward()[1], torch.tensor([0.0, 1.0, 0.0]), atol=1e-6)
+    cam.set_eye(torch.tensor([1.0, 2.0, 3.0]))
+    cam.look_at(torch.tensor([1.0, 2.0, 10.0]) * 0 + torch.tensor([4.0, 6.0, 3.0]))
+    assert torch.allclose(cam.get_forward(), torch.tensor([0.6, 0.8, 0.0]).expand(2, 3), atol=1e-6)
+    V = cam.V_k44[0]
+    assert torch.allclose(V @ torch.tensor([1.0, 2.0, 3.0, 1.0]), torch.tensor([0.0, 0, 0, 1]),
+                          atol=1e-6)
+    with pytest.raises(NotImplementedError):
+        cam.set_positions(torch.zeros(3), keep_lookat=True)
+    with pytest.raises(ValueError):
+        cam.set_eye(torch.zeros(3, 3))  # three rows for two scenes
+
+
+@pytest.mark.parametrize("kw,tiles,tile,window,n", [
+    (dict(num_scenes=10), (4, 3), (64, 64), (256, 192), 10),
+    (dict(num_scenes=4098, tile_resolution=(64, 64)), (65, 64), (64, 64), (4160, 4096), 4098),
+    (dict(tiles=(2, 3), window_resolution=(100, 90)), (2, 3), (50, 30), (100, 90), 6),
+    (dict(tile_resolution=(40, 20), window_resolution=(120, 60)), (3, 3), (40, 20), (120, 60), 9),
+    (dict(tile_resolution=(40, 20)), (1, 1), (40, 20), (40, 20), 1),
+    (dict(window_resolution=(33, 17)), (1, 1), (33, 17), (33, 17), 1),
+    (dict(), (1, 1), (64, 64), (64, 64), 1),
+    (dict(tiles=5), (3, 2), (64, 64), (192, 128), 6),
+])
+def test_atlas_layout_rules(kw, tiles, tile, window, n):
+    """Atlas layout (reference config.py:61-126): grid ceil(sqrt(n)) columns,
+    window = tiles x tile, defaults 64 x 64 tiles and a 1 x 1 grid."""
+    c = PBRConfig(device="cpu", **kw)
+    assert tuple(c.tiles) == tiles and tuple(c.tile_resolution) == tile
+    assert tuple(c.window_resolution) == window and c.num_scenes == n
+    assert c.batch_inner_dim == tiles[0] * tiles[1]
+
+
+def test_atlas_layout_errors():
+    with pytest.raises(ValueError):
+        PBRConfig(device="cpu", tiles=(2, 2), tile_resolution=(8, 8), window_resolution=(16, 17))
+    with pytest.raises(ValueError):
+        PBRConfig(device="cpu", num_scenes=5, tiles=(2, 2))
+    with pytest.raises(ValueError):
+        PBRConfig(device="tpu")
+    c = PBRConfig.from_config({"num_scenes": 3}, device="cpu")
+    assert c.num_scenes == 3 and c.device == "cpu"
+    from pybatchrender.envs.cartpole import CartPoleConfig
+    cc = PBRConfig.from_config(CartPoleConfig(num_scenes=2, device="cpu"), max_steps=7)
+    assert type(cc) is CartPoleConfig and cc.max_steps == 7 and cc.num_scenes == 2
