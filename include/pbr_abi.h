@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define PBR_ABI_VERSION 3
+#define PBR_ABI_VERSION 4
 #define PBR_MAX_NODES 16
 #define PBR_MAX_MESHES 16
 /* Largest tile edge in pixels: window positions in 1/256 pixel then stay
@@ -128,7 +128,8 @@ typedef struct pbr_render_opts {
   int32_t bin_width;          /* scene: 16, 32 or 64 pixel-wide bins        */
   int32_t bins_per_item;      /* scene: bins one work item walks, 0 = auto  */
   int32_t lane_pixels;        /* scene: 4 or 8 pixels per lane, 0 = auto    */
-  int32_t tile_size;          /* tiles: 16, 32 or 64 pixel tiles, 0 = 32    */
+  int32_t tile_size;          /* tiles: 8, 16, 32 or 64 pixel square tiles,
+                                 0 = full-width bands of <= 8192 pixels     */
   int64_t slot_budget;        /* tiles: upper bound of the batch's triangle
                                  slots (sum over scenes of active instances
                                  x triangles); 0 = num_scenes x the largest
@@ -136,7 +137,10 @@ typedef struct pbr_render_opts {
   int32_t row_list_cap;       /* rows: row-list entries per band, 8..256,
                                  0 = 256 (smaller values split the frame
                                  into more bands)                           */
-  int32_t reserved;
+  int32_t bin_capacity;       /* tiles: entries of each of a tile's two LDS
+                                 bins (small and large triangles) before
+                                 they are rasterised and emptied, 1024..
+                                 16384, 0 = 1904 (two workgroups per CU)    */
 } pbr_render_opts;
 
 /* One pose component: value of row i = src ? src[i * stride] : value. */

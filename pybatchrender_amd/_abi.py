@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-PBR_ABI_VERSION = 3
+PBR_ABI_VERSION = 4
 PBR_MAX_NODES = 16
 PBR_MAX_MESHES = 16
 PBR_MAX_RESOLUTION = 4096
@@ -95,7 +95,7 @@ class pbr_render_opts(C.Structure):
         ("tile_size", C.c_int32),
         ("slot_budget", C.c_int64),
         ("row_list_cap", C.c_int32),
-        ("reserved", C.c_int32),
+        ("bin_capacity", C.c_int32),
     ]
 
 
@@ -189,6 +189,7 @@ def expected_layout() -> list[int]:
         pbr_render_opts.bins_per_item.offset,
         pbr_render_opts.slot_budget.offset,
         pbr_render_opts.row_list_cap.offset,
+        pbr_render_opts.bin_capacity.offset,
         C.sizeof(pbr_pose_component),
         C.sizeof(pbr_pose_desc),
         pbr_pose_desc.x.offset,

@@ -6,10 +6,13 @@ counts, how uneven they are, output dtype).  ``autotune(renderer)`` times the
 candidate shapes on the renderer's current scenes, keeps the fastest in the
 renderer's config and returns the measurements:
 
+  row-span kernel (<= 64 triangle slots per scene, <= 128 columns)
   scene kernel (<= 64 triangle slots per scene): bin width 16 / 32 / 64,
       4 or 8 pixels per lane, bins per work item 0 (auto) / 1 / 2 / 4
-  tile kernels: tile 0 (full-width bands) / 16 / 32 / 64, with and without
-      the persistent (scene, tile) queue, 4 / 8 / 16 workgroups per CU
+  tile kernels: tile 0 (full-width bands) / 8 / 16 / 32 / 64 pixels (8 only
+      for scenes of at most 2048 triangle slots: every tile tests every
+      slot's box), LDS bin capacity 1024 / 1904 / 4096 entries, and the
+      persistent (scene, tile) queue at 4 / 8 / 16 workgroups per CU
 
 Results are cached per batch signature in memory (and in ``cache_path`` as
 JSON when one is given), so a later renderer with the same signature reuses
@@ -24,7 +27,8 @@ from pathlib import Path
 
 import torch
 
-_FIELDS = ("kernel", "tile_size", "persistent", "bin_width", "bins_per_item", "lane_pixels")
+_FIELDS = ("kernel", "tile_size", "persistent", "bin_width", "bins_per_item", "lane_pixels",
+           "bin_capacity")
 _CACHE: dict[str, dict] = {}
 
 
@@ -56,8 +60,11 @@ def candidates(renderer) -> list[dict]:
                 for per in (0, 1, 2, 4):
                     out.append({"kernel": "scene", "bin_width": bw, "lane_pixels": px,
                                 "bins_per_item": per})
-    for ts in (0, 16, 32, 64):
-        out.append({"kernel": "tiles", "tile_size": ts, "persistent": False})
+    sizes = (0, 8, 16, 32, 64) if max_slots(renderer) <= 2048 else (0, 16, 32, 64)
+    for ts in sizes:
+        for cap in (1024, 0, 4096):
+            out.append({"kernel": "tiles", "tile_size": ts, "persistent": False,
+                        "bin_capacity": cap})
         for wpc in (4, 8, 16):
             out.append({"kernel": "tiles", "tile_size": ts, "persistent": True,
                         "workgroups_per_cu": wpc})
@@ -72,6 +79,7 @@ def _apply(renderer, cand: dict) -> None:
     cfg.bin_width = int(cand.get("bin_width", 0))
     cfg.bins_per_item = int(cand.get("bins_per_item", 0))
     cfg.lane_pixels = int(cand.get("lane_pixels", 0))
+    cfg.bin_capacity = int(cand.get("bin_capacity", 0))
     wpc = int(cand.get("workgroups_per_cu", 0))
     if wpc:
         cus = torch.cuda.get_device_properties(renderer.device).multi_processor_count

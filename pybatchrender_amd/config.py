@@ -17,7 +17,10 @@ New fields (defaults keep the reference's behaviour):
                    64 triangle slots per scene), 'tiles' (any triangle count:
                    one setup pass, then one workgroup per scene tile) or
                    'chunked' (any triangle count, setup repeated per row strip)
-    tile_size      tile kernels: 16, 32 or 64 pixel square tiles (0 = 32)
+    tile_size      tile kernels: 8, 16, 32 or 64 pixel square tiles (0 =
+                   full-width bands of at most 8192 pixels)
+    bin_capacity   tile kernels: entries of each LDS bin of a tile before it
+                   is rasterised and emptied, 1024..16384 (0 = 1904)
     row_list_cap   rows kernel: row-list entries per band, 8..256 (0 = 256)
     bin_width      scene kernel: 16, 32 or 64 pixel-wide bins (0 = automatic)
     bins_per_item  scene kernel: bins walked by one work item (0 = automatic)
@@ -91,6 +94,7 @@ class PBRConfig:
     lane_pixels: int = 0
     tile_size: int = 0
     row_list_cap: int = 0
+    bin_capacity: int = 0
 
     def process_resolution(self) -> None:
         """Atlas layout (reference config.py:61-126).  An int ``tiles`` (or
@@ -139,8 +143,10 @@ class PBRConfig:
             raise ValueError("region_rows must be 0 or a positive multiple of 8")
         if self.kernel not in ("auto", "chunked", "scene", "tiles", "rows"):
             raise ValueError(f"Invalid kernel: {self.kernel}")
-        if self.tile_size not in (0, 16, 32, 64):
-            raise ValueError("tile_size must be 0, 16, 32 or 64")
+        if self.tile_size not in (0, 8, 16, 32, 64):
+            raise ValueError("tile_size must be 0, 8, 16, 32 or 64")
+        if self.bin_capacity != 0 and not 1024 <= self.bin_capacity <= 16384:
+            raise ValueError("bin_capacity must be 0 or within [1024, 16384]")
         if self.row_list_cap != 0 and not 8 <= self.row_list_cap <= 256:
             raise ValueError("row_list_cap must be 0 or within [8, 256]")
         if self.bin_width not in (0, 16, 32, 64):

@@ -1162,6 +1162,7 @@ int pbr_abi_layout(int64_t* out, int32_t n) {
       (int64_t)offsetof(pbr_render_opts, bins_per_item),
       (int64_t)offsetof(pbr_render_opts, slot_budget),
       (int64_t)offsetof(pbr_render_opts, row_list_cap),
+      (int64_t)offsetof(pbr_render_opts, bin_capacity),
       (int64_t)sizeof(pbr_pose_component),
       (int64_t)sizeof(pbr_pose_desc),
       (int64_t)offsetof(pbr_pose_desc, x),

@@ -40,11 +40,13 @@ namespace {
 constexpr int kTileBlock = 512;
 constexpr int kPerThread = 2;                        // boxes one thread tests per round
 constexpr int kRound = kTileBlock * kPerThread;      // boxes per round
-// entries per LDS queue: a round adds at most kRound, so a queue holding more
-// than kQCap - kRound entries is flushed before the next round.  1904: 64 KiB
-// of pixel keys, both queues, the record staging slots and the static LDS
-// then stay under half of the CU's 160 KiB, so two workgroups share a CU.
+// entries per LDS queue (bin) by default: a round adds at most kRound, so a
+// queue holding more than qcap - kRound entries is flushed before the next
+// round.  1904: 64 KiB of pixel keys, both queues, the record staging slots
+// and the static LDS then stay under half of the CU's 160 KiB, so two
+// workgroups share a CU.  pbr_render_opts.bin_capacity sets other sizes.
 constexpr int kQCap = 2 * kRound - 144;
+constexpr int kMaxQCap = 16384;
 constexpr int kAutoTilePixels = 8192;                // auto tiles: 64 KiB of pixel state
 constexpr int kSmallArea = 48;   // box-in-tile pixels one lane rasterises alone
 constexpr int kResolvePx = 4;    // pixels a thread resolves per pass (flat shading)
@@ -58,6 +60,7 @@ struct TileFrame {
   int shw;                   // floats per record in `shades` (4 or 16)
   uint32_t budget;           // records the pool holds
   int float_out;             // 1: bf16 / float32 colour output
+  int qcap;                  // entries of each LDS queue (>= kRound)
 };
 
 struct TilePool {
@@ -446,7 +449,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void tile_raster_kernel(pbr_scene_desc D
                                                                  int persistent) {
   extern __shared__ unsigned long long pix[];
   uint32_t* qs = reinterpret_cast<uint32_t*>(pix + F.tw * F.th);
-  uint32_t* qb = qs + kQCap;
+  uint32_t* qb = qs + F.qcap;
   __shared__ uint32_t ns, nb, item_sh;
   __shared__ uint32_t base[PBR_MAX_NODES + 1];
   __shared__ __align__(16) uint4 stage[kTileBlock / 64][2][4];  // per wave: two records
@@ -516,8 +519,8 @@ __attribute__((amdgpu_waves_per_eu(4))) void tile_raster_kernel(pbr_scene_desc D
       __syncthreads();
       const bool last = r0 + kRound >= n;
       const uint32_t cs = ns, cb = nb;
-      const bool flush_s = cs > (uint32_t)(kQCap - kRound) || (last && cs > 0);
-      const bool flush_b = cb > (uint32_t)(kQCap - kRound) || (last && cb > 0);
+      const bool flush_s = cs > (uint32_t)(F.qcap - kRound) || (last && cs > 0);
+      const bool flush_b = cb > (uint32_t)(F.qcap - kRound) || (last && cb > 0);
       if (flush_s) {
         for (uint32_t k = (uint32_t)tid; k < cs; k += kTileBlock) {
           const uint32_t sl = qs[k];
@@ -744,8 +747,11 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 int tiles_plan(const pbr_scene_desc* D, const pbr_render_opts* opts, TilePlan& T) {
   memset(&T, 0, sizeof(T));
   const int ts = (opts && opts->tile_size > 0) ? opts->tile_size : 0;
-  if (ts != 0 && ts != 16 && ts != 32 && ts != 64)
-    return fail(PBR_ERR_INVALID, "tile_size %d is not 16, 32 or 64", ts);
+  if (ts != 0 && ts != 8 && ts != 16 && ts != 32 && ts != 64)
+    return fail(PBR_ERR_INVALID, "tile_size %d is not 8, 16, 32 or 64", ts);
+  const int qcap = (opts && opts->bin_capacity > 0) ? opts->bin_capacity : kQCap;
+  if (qcap < kRound || qcap > kMaxQCap)
+    return fail(PBR_ERR_INVALID, "bin_capacity %d outside [%d, %d]", qcap, kRound, kMaxQCap);
   const uint64_t ms = max_slots(D);
   if (ms >= kMaxTileSlots)
     return fail(PBR_ERR_UNSUPPORTED, "tile path: %llu triangle slots per scene (limit 2^29)",
@@ -770,7 +776,12 @@ int tiles_plan(const pbr_scene_desc* D, const pbr_render_opts* opts, TilePlan& T
   }
   F.ntx = (D->width + F.tw - 1) / F.tw;
   F.tiles = F.ntx * ((D->height + F.th - 1) / F.th);
-  T.lds = sizeof(unsigned long long) * (size_t)F.tw * F.th + 2 * sizeof(uint32_t) * kQCap;
+  F.qcap = qcap;
+  T.lds = sizeof(unsigned long long) * (size_t)F.tw * F.th + 2 * sizeof(uint32_t) * (size_t)qcap;
+  // the kernel's static LDS (counters, node table, record staging) is < 2 KiB
+  if (T.lds + 2048 > 160 * 1024)
+    return fail(PBR_ERR_UNSUPPORTED, "tile of %d x %d pixels with bins of %d entries needs %zu "
+                "bytes of LDS", F.tw, F.th, qcap, T.lds);
   F.guard = 65536.0f / (float)(D->width > D->height ? D->width : D->height);
   F.max_slots = (uint32_t)ms;
   F.uniform = T.counts ? 0 : 1;

@@ -322,7 +322,8 @@ class PBRRenderer:
     def _opts_key(self) -> tuple:
         c = self.cfg
         return (c.region_rows, c.persistent, getattr(self, "workgroups", 0), c.kernel, c.bin_width,
-                c.bins_per_item, c.lane_pixels, c.tile_size, c.row_list_cap, self._desc_cache[3])
+                c.bins_per_item, c.lane_pixels, c.tile_size, c.row_list_cap, c.bin_capacity,
+                self._desc_cache[3])
 
     def _cached_opts(self) -> _abi.pbr_render_opts:
         """render_opts() for the current configuration, rebuilt when a field
@@ -349,6 +350,7 @@ class PBRRenderer:
         o.tile_size = int(self.cfg.tile_size)
         o.slot_budget = self.slot_budget()
         o.row_list_cap = int(self.cfg.row_list_cap)
+        o.bin_capacity = int(self.cfg.bin_capacity)
         return o
 
     def slot_budget(self) -> int:

@@ -138,7 +138,7 @@ def test_persistent_queue(cuda, kernel):
     check_parity(r, f"persistent work queue ({kernel}, second launch)")
 
 
-@pytest.mark.parametrize("ts", [16, 32, 64])
+@pytest.mark.parametrize("ts", [8, 16, 32, 64])
 @pytest.mark.parametrize("res", [(84, 84), (128, 72), (50, 30)])
 def test_tile_sizes(cuda, ts, res):
     gen = torch.Generator("cpu").manual_seed(13)
@@ -157,6 +157,16 @@ def test_tile_sizes(cuda, ts, res):
     r.add_light()
     r.setup_environment()
     check_parity(r, f"tiles {ts} at {res}")
+
+
+@pytest.mark.parametrize("cap,ts", [(1024, 0), (1024, 8), (4096, 32), (16384, 16)])
+def test_tile_bin_capacities(cuda, cap, ts):
+    """LDS bins of 1024 entries (emptied after every round of box tests) up
+    to 16384: up to 256 instances per scene fill the bins of a tile many
+    times over."""
+    r = workloads.uneven_batch(num_scenes=24, resolution=64, max_instances=256, seed=4,
+                               kernel="tiles", tile_size=ts, bin_capacity=cap)
+    check_parity(r, f"bin capacity {cap}, tile {ts}")
 
 
 def test_tiles_uneven_batch_full_size_slice(cuda):
@@ -562,10 +572,16 @@ def test_batched_poses_match_single(cuda):
 
 def test_autotune_picks_a_shape_that_matches_the_oracle(cuda, tmp_path):
     from pybatchrender import autotune
-    r = workloads.uneven_batch(num_scenes=48, resolution=84, max_instances=32, seed=21)
+    # boxes and spheres, at most 8 each: 1888 slots per scene, so 8-pixel
+    # tiles are among the candidates
+    r = workloads.uneven_batch(num_scenes=48, resolution=84, max_instances=8, seed=21,
+                               models=("models/box", "models/sphere"))
     cache = tmp_path / "tune.json"
     rep = autotune.autotune(r, launches=2, cache_path=cache)
     assert not rep["cached"] and len(rep["results"]) >= 16
+    tried = [c["cand"] for c in rep["results"]]
+    assert any(c.get("tile_size") == 8 for c in tried)
+    assert {c.get("bin_capacity") for c in tried if c["kernel"] == "tiles"} >= {1024, 4096}
     print(f"[autotune] best {rep['best']} {rep['us']} us", flush=True)
     check_parity(r, f"autotuned {rep['best']}")
     rep2 = autotune.autotune(r, cache_path=cache)

@@ -273,12 +273,19 @@ def test_autotune_candidates_and_signature():
     r = CartPoleGroundRenderer(CartPoleConfig(num_scenes=8, device="cpu"))
     cands = autotune.candidates(r)
     assert any(c["kernel"] == "scene" for c in cands)  # 26 slots: the scene kernel qualifies
-    assert {c.get("tile_size") for c in cands if c["kernel"] == "tiles"} == {0, 16, 32, 64}
+    tiles = [c for c in cands if c["kernel"] == "tiles"]
+    assert {c.get("tile_size") for c in tiles} == {0, 8, 16, 32, 64}
+    assert {c.get("bin_capacity") for c in tiles if not c["persistent"]} == {0, 1024, 4096}
     sig = autotune.signature(r)
     assert '"N": 8' in sig and '"W": 64' in sig
-    autotune._apply(r, {"kernel": "tiles", "tile_size": 32, "persistent": False})
+    autotune._apply(r, {"kernel": "tiles", "tile_size": 8, "persistent": False,
+                        "bin_capacity": 4096})
     o = r.render_opts()
-    assert o.kernel == 3 and o.tile_size == 32 and o.persistent == 0
+    assert o.kernel == 3 and o.tile_size == 8 and o.persistent == 0 and o.bin_capacity == 4096
+    # 8-pixel tiles only while every tile can test every slot cheaply
+    from pybatchrender import workloads
+    big = workloads.uneven_batch(num_scenes=4, resolution=32, max_instances=64, device="cpu")
+    assert 8 not in {c.get("tile_size") for c in autotune.candidates(big)}
 
 
 def test_camera_setters_closed_forms():
