@@ -33,6 +33,10 @@ Printed JSON line (one, from this single process):
                render, bf16 RGB + depth), one hipGraph replay per step
   env_step     CartPole-v0 (reference scene) env.step at the same size:
                dynamics + auto-reset + render, scene-frames/s
+  tile_workloads  many-cubes (1024 scenes 128x128 float32 RGB + depth) and
+               the uneven persistent batch (8192 scenes 84x84, 1-256
+               instances) vs its uniform twin: render-call time (HIP
+               events), HBM rate and the uneven/uniform ns-per-triangle ratio
 """
 from __future__ import annotations
 
@@ -372,6 +376,51 @@ def env_loop_leg(n: int = 16384, steps: int = 1000) -> dict:
     }
 
 
+def tile_workloads_leg(launches: int = 30) -> dict:
+    """The many-cubes batch (1024 scenes 128x128, float32 RGB + depth, 64
+    boxes each) and the uneven persistent batch (8192 scenes 84x84 uint8,
+    1-256 instances of box / sphere / cylinder) against its uniform twin
+    (every scene at the mean instance count): render-call time from HIP
+    events, algorithmic HBM rate, and ns per triangle."""
+    import torch
+
+    from pybatchrender import workloads
+
+    out = {}
+    r = workloads.many_cubes()
+    kus = kernel_time_us(r, launches=launches)
+    nb = algorithmic_bytes(r)
+    out["many_cubes"] = {
+        "workload": "1024 scenes 128x128 float32 RGB + float32 depth, 64 random boxes each",
+        "kernel": render_kernel_name(r), "render_us": round(kus, 2),
+        "scene_frames_per_s": round(r.num_scenes / kus * 1e6, 1),
+        "bytes_per_launch": nb, "achieved_gbs": round(nb / kus / 1e3, 1),
+        "frac": round(nb / kus / 1e3 / HBM_PEAK_GBS, 4),
+        "triangles": triangles_drawn(r)}
+    del r
+    torch.cuda.empty_cache()
+    res = {}
+    for name, uniform in (("uneven", False), ("uniform", True)):
+        r = workloads.uneven_batch(persistent=True, uniform=uniform)
+        kus = kernel_time_us(r, launches=max(5, launches // 3))
+        tri = triangles_drawn(r)
+        nb = algorithmic_bytes(r)
+        res[name] = {"render_us": round(kus, 2), "triangles": tri,
+                     "ns_per_triangle": round(kus * 1e3 / tri, 5),
+                     "scene_frames_per_s": round(r.num_scenes / kus * 1e6, 1),
+                     "achieved_gbs": round(nb / kus / 1e3, 1), "kernel": render_kernel_name(r)}
+        del r
+        torch.cuda.empty_cache()
+    out["uneven_persistent"] = {
+        "workload": "8192 scenes 84x84 uint8, log-uniform 1-256 instances of box / UV sphere / "
+                    "cylinder, one persistent (scene, tile) work-queue launch; uniform: same "
+                    "draws with every scene at the mean instance count",
+        **res,
+        "uneven_over_uniform_per_triangle": round(res["uneven"]["ns_per_triangle"] /
+                                                  res["uniform"]["ns_per_triangle"], 3)}
+    return out
+
+
 def render_kernel_name(r) -> str:
     """The kernel family pbr_render runs for this renderer (pbr_choose_kernel)."""
     return r._chosen_kernel()
@@ -491,6 +540,8 @@ def main() -> int:
     ap.add_argument("--no-env", action="store_true")
     ap.add_argument("--no-bwd", action="store_true")
     ap.add_argument("--no-loop", action="store_true")
+    ap.add_argument("--no-tiles", action="store_true",
+                    help="skip the many-cubes and uneven-batch legs")
     ap.add_argument("--loop-scenes", type=int, default=16384)
     ap.add_argument("--loop-steps", type=int, default=1000)
     ap.add_argument("--kernel-only", action="store_true",
@@ -620,6 +671,9 @@ def main() -> int:
     if not args.no_loop:
         log("[bench] on-device env loop")
         rec["env_loop"] = env_loop_leg(args.loop_scenes, args.loop_steps)
+    if not args.no_tiles:
+        log("[bench] many-cubes and uneven persistent batches")
+        rec["tile_workloads"] = tile_workloads_leg()
     if not args.no_env:
         log("[bench] env.step CartPole-v0")
         es = env_step_rate(n, 50)
