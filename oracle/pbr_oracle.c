@@ -93,9 +93,10 @@ static void normalize3(const float v[3], float out[3]) {
   d = d + v[2] * v[2];
   float r = sqrtf(d);
   if (r > 0.0f) {
-    out[0] = v[0] / r;
-    out[1] = v[1] / r;
-    out[2] = v[2] / r;
+    const float inv = 1.0f / r; /* v * (1 / |v|): the kernels' rounding */
+    out[0] = v[0] * inv;
+    out[1] = v[1] * inv;
+    out[2] = v[2] * inv;
   } else {
     out[0] = out[1] = out[2] = 0.0f;
   }
@@ -192,14 +193,15 @@ static void raster_tri(target_t* T, const cvert* v0, const cvert* v1, const cver
   for (int k = 0; k < 3; ++k) {
     float w = v[k]->p[3];
     if (!(w > 0.0f)) return;
-    float nx = v[k]->p[0] / w, ny = v[k]->p[1] / w, nz = v[k]->p[2] / w;
+    const float iw = 1.0f / w; /* perspective divide as x * (1 / w) */
+    float nx = v[k]->p[0] * iw, ny = v[k]->p[1] * iw, nz = v[k]->p[2] * iw;
     float sx = (nx + 1.0f) * (0.5f * (float)W);
     float sy = (ny + 1.0f) * (0.5f * (float)H);
     if (!(fabsf(sx) < 65536.0f) || !(fabsf(sy) < 65536.0f)) return; /* non-finite input */
     X[k] = (int64_t)rintf(sx * (float)SUBPIX);
     Y[k] = (int64_t)rintf(sy * (float)SUBPIX);
     d[k] = nz * 0.5f + 0.5f;
-    invw[k] = 1.0f / w;
+    invw[k] = iw;
   }
   int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
   if (area <= 0) return; /* clockwise (back face) or degenerate */

@@ -28,14 +28,16 @@ struct Light {
   float dir[3], col[3], amb[3], s;
 };
 
+// v / |v| as v * (1 / |v|): one IEEE division instead of three.
 __device__ __forceinline__ void normalize3(const float v[3], float out[3]) {
   float d = v[0] * v[0] + v[1] * v[1];
   d = d + v[2] * v[2];
   float r = sqrtf(d);
   if (r > 0.0f) {
-    out[0] = v[0] / r;
-    out[1] = v[1] / r;
-    out[2] = v[2] / r;
+    const float inv = 1.0f / r;
+    out[0] = v[0] * inv;
+    out[1] = v[1] * inv;
+    out[2] = v[2] * inv;
   } else {
     out[0] = out[1] = out[2] = 0.0f;
   }
@@ -110,8 +112,9 @@ inline Light host_light(const pbr_scene_desc* D) {
   float d = v[0] * v[0] + v[1] * v[1];
   d = d + v[2] * v[2];
   const float r = sqrtf(d);
+  const float inv = r > 0.0f ? 1.0f / r : 0.0f;
   for (int c = 0; c < 3; ++c) {
-    lt.dir[c] = r > 0.0f ? v[c] / r : 0.0f;
+    lt.dir[c] = r > 0.0f ? v[c] * inv : 0.0f;
     lt.col[c] = D->light_col[c];
     lt.amb[c] = D->ambient[c];
   }
@@ -284,14 +287,16 @@ __device__ __forceinline__ void snap_vertex(const CVert& v, int W, int H, SnapVe
   S.d = S.invw = 0.0f;
   float w = v.p[3];
   if (!(w > 0.0f)) return;
-  float nx = v.p[0] / w, ny = v.p[1] / w, nz = v.p[2] / w;
+  // perspective divide by one reciprocal (as GPUs divide): x * (1 / w)
+  const float iw = 1.0f / w;
+  float nx = v.p[0] * iw, ny = v.p[1] * iw, nz = v.p[2] * iw;
   float sx = (nx + 1.0f) * (0.5f * (float)W);
   float sy = (ny + 1.0f) * (0.5f * (float)H);
   if (!(fabsf(sx) < 65536.0f) || !(fabsf(sy) < 65536.0f)) return;  // non-finite input
   S.X = (int32_t)rintf(sx * 256.0f);
   S.Y = (int32_t)rintf(sy * 256.0f);
   S.d = nz * 0.5f + 0.5f;
-  S.invw = 1.0f / w;
+  S.invw = iw;
   S.ok = true;
 }
 
@@ -302,40 +307,38 @@ __device__ __forceinline__ bool setup_snapped(const SnapVert& s0, const SnapVert
                                               const SnapVert& s2, int W, int H, int jr0, int jr1,
                                               FixTri& T) {
   if (!(s0.ok && s1.ok && s2.ok)) return false;
-  const SnapVert* sv[3] = {&s0, &s1, &s2};
-  int64_t X[3], Y[3];
-  float d[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    X[k] = sv[k]->X;
-    Y[k] = sv[k]->Y;
-    d[k] = sv[k]->d;
-    T.invw[k] = sv[k]->invw;
-  }
-  int64_t area = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
-  if (area <= 0) return false;
-  int64_t xmin = min(X[0], min(X[1], X[2])), xmax = max(X[0], max(X[1], X[2]));
-  int64_t ymin = min(Y[0], min(Y[1], Y[2])), ymax = max(Y[0], max(Y[1], Y[2]));
-  int64_t i0 = (xmin - 128 + 255) >> 8, i1 = (xmax - 128) >> 8;
-  int64_t j0 = (ymin - 128 + 255) >> 8, j1 = (ymax - 128) >> 8;
-  i0 = max(i0, (int64_t)0);
-  i1 = min(i1, (int64_t)(W - 1));
-  j0 = max(j0, (int64_t)jr0);
-  j1 = min(j1, (int64_t)jr1);
+  // Snapped positions satisfy |X|, |Y| <= 2^24 (snap_vertex rejects larger
+  // ones), so coordinate differences fit 25 bits, their products are below
+  // 2^50 and exact in double, and so is the difference of two products:
+  // 32-bit integer and double arithmetic give the exact integer results of
+  // the 64-bit integer forms, at a fraction of their instructions.
+  const int32_t X[3] = {s0.X, s1.X, s2.X}, Y[3] = {s0.Y, s1.Y, s2.Y};
+  const float d[3] = {s0.d, s1.d, s2.d};
+  T.invw[0] = s0.invw;
+  T.invw[1] = s1.invw;
+  T.invw[2] = s2.invw;
+  const double area = (double)(X[1] - X[0]) * (double)(Y[2] - Y[0]) -
+                      (double)(X[2] - X[0]) * (double)(Y[1] - Y[0]);
+  if (!(area > 0.0)) return false;
+  const int32_t xmin = min(X[0], min(X[1], X[2])), xmax = max(X[0], max(X[1], X[2]));
+  const int32_t ymin = min(Y[0], min(Y[1], Y[2])), ymax = max(Y[0], max(Y[1], Y[2]));
+  // pixel centres 256 i + 128 inside [min, max]: floor division by 256
+  const int32_t i0 = max((xmin - 128 + 255) >> 8, 0), i1 = min((xmax - 128) >> 8, W - 1);
+  const int32_t j0 = max((ymin - 128 + 255) >> 8, jr0), j1 = min((ymax - 128) >> 8, jr1);
   if (i0 > i1 || j0 > j1) return false;
-  T.i0 = (int)i0;
-  T.i1 = (int)i1;
-  T.j0 = (int)j0;
-  T.j1 = (int)j1;
+  T.i0 = i0;
+  T.i1 = i1;
+  T.j0 = j0;
+  T.j1 = j1;
   uint32_t bias = 0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int a = (k + 1) % 3, b = (k + 2) % 3;
-    const int64_t dx = X[b] - X[a], dy = Y[b] - Y[a];
+    const int32_t dx = X[b] - X[a], dy = Y[b] - Y[a];
     const bool tl = (dy < 0) || (dy == 0 && dx < 0);
     if (!tl) bias |= 1u << k;
-    T.X[k] = (int32_t)X[k];
-    T.Y[k] = (int32_t)Y[k];
+    T.X[k] = X[k];
+    T.Y[k] = Y[k];
   }
   T.bias = bias;
   double x0 = (double)X[0] * (1.0 / 256.0), y0 = (double)Y[0] * (1.0 / 256.0);
@@ -362,6 +365,71 @@ __device__ __forceinline__ bool setup_fixed(const CVert& v0, const CVert& v1, co
   snap_vertex(v1, W, H, s1);
   snap_vertex(v2, W, H, s2);
   return setup_snapped(s0, s1, s2, W, H, jr0, jr1, T);
+}
+
+// Vertex record: one (instance, mesh vertex) after (a) and the per-vertex
+// part of (b), so that each is transformed and snapped once however many
+// triangles share it.
+struct VertRec {
+  int32_t X, Y;
+  float d, invw;
+  float n[3];     // normalised world normal
+  uint32_t bits;  // bit 0: snapped; bit 1 + p: inside plane p; bit 7 + p: outside plane p
+};
+static_assert(sizeof(VertRec) == 32, "VertRec is 32 bytes");
+
+// (a) + per-vertex (b) of model-space vertex vv (px py pz nx ny nz) under
+// instance matrix M and the scene's VP: the same products, snap and plane
+// distances fetch_node_triangle, snap_vertex and clip_class evaluate.
+__device__ __forceinline__ void vertex_record(const float M[16], const float VP[16],
+                                              const float* vv, int W, int H, float guard,
+                                              VertRec& out) {
+  CVert cv;
+  float wp[4], nw[3];
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    float a = M[rr * 4 + 0] * vv[0] + M[rr * 4 + 1] * vv[1];
+    a = a + M[rr * 4 + 2] * vv[2];
+    wp[rr] = a + M[rr * 4 + 3];
+  }
+#pragma unroll
+  for (int rr = 0; rr < 4; ++rr) {
+    float a = VP[rr * 4 + 0] * wp[0] + VP[rr * 4 + 1] * wp[1];
+    a = a + VP[rr * 4 + 2] * wp[2];
+    cv.p[rr] = a + VP[rr * 4 + 3] * wp[3];
+  }
+#pragma unroll
+  for (int rr = 0; rr < 3; ++rr) {
+    float a = M[rr * 4 + 0] * vv[3] + M[rr * 4 + 1] * vv[4];
+    nw[rr] = a + M[rr * 4 + 2] * vv[5];
+  }
+  normalize3(nw, cv.n);
+  SnapVert sv;
+  snap_vertex(cv, W, H, sv);
+  uint32_t bits = sv.ok ? 1u : 0u;
+#pragma unroll
+  for (int pl = 0; pl < 6; ++pl) {
+    const float dpl = plane_dist(cv, pl, guard);
+    if (dpl >= 0.0f) bits |= 2u << pl;
+    if (dpl < 0.0f) bits |= 128u << pl;
+  }
+  out.X = sv.X;
+  out.Y = sv.Y;
+  out.d = sv.d;
+  out.invw = sv.invw;
+  out.n[0] = cv.n[0];
+  out.n[1] = cv.n[1];
+  out.n[2] = cv.n[2];
+  out.bits = bits;
+}
+
+__device__ __forceinline__ void load_mat4(const float* src, float M[16]) {
+  const float4* q = reinterpret_cast<const float4*>(src);
+  const float4 a = q[0], b = q[1], c = q[2], d = q[3];
+  M[0] = a.x; M[1] = a.y; M[2] = a.z; M[3] = a.w;
+  M[4] = b.x; M[5] = b.y; M[6] = b.z; M[7] = b.w;
+  M[8] = c.x; M[9] = c.y; M[10] = c.z; M[11] = c.w;
+  M[12] = d.x; M[13] = d.y; M[14] = d.z; M[15] = d.w;
 }
 
 // Exact edge functions e_k = A*Px + B*Py + C of a snapped triangle (edge k is

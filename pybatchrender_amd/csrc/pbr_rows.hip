@@ -436,17 +436,6 @@ __device__ __forceinline__ void setup_slot(const pbr_scene_desc& D, const RowsFr
   }
 }
 
-// Vertex table entry: one (instance, mesh vertex) of a scene after (a) and
-// the per-vertex part of (b) - each is transformed and snapped once, however
-// many triangles share it.
-struct VertRec {
-  int32_t X, Y;
-  float d, invw;
-  float n[3];     // normalised world normal
-  uint32_t bits;  // bit 0: snapped; bit 1 + p: inside plane p; bit 7 + p: outside plane p
-};
-static_assert(sizeof(VertRec) == 32, "VertRec is 32 bytes");
-
 // The vertex table of a workgroup's two scenes lives in the (not yet used)
 // row-list space.
 constexpr int kVertCap = (int)((kRowsBlock / 64) * sizeof(ListLds) / sizeof(VertRec));
@@ -480,52 +469,10 @@ __device__ __forceinline__ void vertex_slot(const pbr_scene_desc& D, const RowsF
   if (!found) return;
   const uint32_t i = local / (uint32_t)nv, k = local - i * (uint32_t)nv;
   const size_t row = shared ? (size_t)i : (size_t)s * (size_t)ips + (size_t)i;
-  const float4* M4 = reinterpret_cast<const float4*>(trans + row * 16);
-  const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2], m3 = M4[3];
-  const float M[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
-                       m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
-  const float4* V4 = reinterpret_cast<const float4*>(D.view_proj + (size_t)s * 16);
-  const float4 q0 = V4[0], q1 = V4[1], q2 = V4[2], q3 = V4[3];
-  const float VP[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                        q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-  const float* vv = D.vertices + (size_t)(voff + (int)k) * 6;
-  CVert cv;
-  float wp[4], nw[3];
-#pragma unroll
-  for (int rr = 0; rr < 4; ++rr) {
-    float a = M[rr * 4 + 0] * vv[0] + M[rr * 4 + 1] * vv[1];
-    a = a + M[rr * 4 + 2] * vv[2];
-    wp[rr] = a + M[rr * 4 + 3];
-  }
-#pragma unroll
-  for (int rr = 0; rr < 4; ++rr) {
-    float a = VP[rr * 4 + 0] * wp[0] + VP[rr * 4 + 1] * wp[1];
-    a = a + VP[rr * 4 + 2] * wp[2];
-    cv.p[rr] = a + VP[rr * 4 + 3] * wp[3];
-  }
-#pragma unroll
-  for (int rr = 0; rr < 3; ++rr) {
-    float a = M[rr * 4 + 0] * vv[3] + M[rr * 4 + 1] * vv[4];
-    nw[rr] = a + M[rr * 4 + 2] * vv[5];
-  }
-  normalize3(nw, cv.n);
-  SnapVert sv;
-  snap_vertex(cv, F.W, F.H, sv);
-  uint32_t bits = sv.ok ? 1u : 0u;
-#pragma unroll
-  for (int pl = 0; pl < 6; ++pl) {
-    const float dpl = plane_dist(cv, pl, F.guard);
-    if (dpl >= 0.0f) bits |= 2u << pl;
-    if (dpl < 0.0f) bits |= 128u << pl;
-  }
-  out.X = sv.X;
-  out.Y = sv.Y;
-  out.d = sv.d;
-  out.invw = sv.invw;
-  out.n[0] = cv.n[0];
-  out.n[1] = cv.n[1];
-  out.n[2] = cv.n[2];
-  out.bits = bits;
+  float M[16], VP[16];
+  load_mat4(trans + row * 16, M);
+  load_mat4(D.view_proj + (size_t)s * 16, VP);
+  vertex_record(M, VP, D.vertices + (size_t)(voff + (int)k) * 6, F.W, F.H, F.guard, out);
 }
 
 // (a)+(b) of one slot of scene s from the scene's vertex table `vt`: the
