@@ -367,6 +367,13 @@ __device__ __forceinline__ bool setup_fixed(const CVert& v0, const CVert& v1, co
   return setup_snapped(s0, s1, s2, W, H, jr0, jr1, T);
 }
 
+// Orders a wave's LDS writes before its later LDS reads (wave-level only).
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Vertex record: one (instance, mesh vertex) after (a) and the per-vertex
 // part of (b), so that each is transformed and snapped once however many
 // triangles share it.

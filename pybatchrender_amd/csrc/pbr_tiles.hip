@@ -210,6 +210,31 @@ __device__ __forceinline__ void locate_node(const pbr_scene_desc& D, int s, uint
   }
 }
 
+// Record of a slot whose triangle straddles a clip plane: the union box of
+// its sub-triangles (re-derived where they are drawn).
+template <bool SMOOTH>
+__device__ __forceinline__ void setup_clipped(const TileFrame& F, const TilePool& P, uint32_t g,
+                                              uint32_t slot, const CVert cv[3], bool flat,
+                                              const float flat_rgba[4], float* sh, short4& bb) {
+  if (!clipped_box(cv, F, bb)) return;
+  GRec r = {};
+  r.id = slot;
+  r.i0 = bb.x;
+  r.i1 = bb.y;
+  r.j0 = bb.z;
+  r.j1 = bb.w;
+  const bool smooth = SMOOTH && !flat;
+  r.flags = kFlagClipped | (smooth ? kFlagSmooth : 0u);
+  r.color = smooth ? 0u : pack_rgba8(flat_rgba);
+  const uint4* src = reinterpret_cast<const uint4*>(&r);
+  uint4* dst = reinterpret_cast<uint4*>(P.recs + g);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[q] = src[q];
+  if (!smooth && sh)
+    *reinterpret_cast<float4*>(sh) =
+        make_float4(flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]);
+}
+
 // Setup of pool slot g (scene slot `slot`): box, record, colours.
 template <bool SMOOTH>
 __device__ __forceinline__ void tile_setup_one(const TileFrame& F, const TilePool& P,
@@ -231,52 +256,135 @@ __device__ __forceinline__ void tile_setup_one(const TileFrame& F, const TilePoo
       bb = make_short4((short)T.i0, (short)T.i1, (short)T.j0, (short)T.j1);
     }
   } else if (!all_out) {
-    if (clipped_box(cv, F, bb)) {
-      GRec r = {};
-      r.id = slot;
-      r.i0 = bb.x;
-      r.i1 = bb.y;
-      r.j0 = bb.z;
-      r.j1 = bb.w;
-      const bool smooth = SMOOTH && !flat;
-      r.flags = kFlagClipped | (smooth ? kFlagSmooth : 0u);
-      r.color = smooth ? 0u : pack_rgba8(flat_rgba);
-      const uint4* src = reinterpret_cast<const uint4*>(&r);
-      uint4* dst = reinterpret_cast<uint4*>(P.recs + g);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) dst[q] = src[q];
-      if (!smooth && sh)
-        *reinterpret_cast<float4*>(sh) =
-            make_float4(flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]);
-    }
+    setup_clipped<SMOOTH>(F, P, g, slot, cv, flat, flat_rgba, sh, bb);
   }
   P.box[g] = bb;
   if (!F.float_out) P.col8[g] = pack_rgba8(flat_rgba);
 }
 
-// Grid-stride over the pool: thread -> pool slot g -> (scene, slot).
+// The setup of slot g from its instance's vertex records `vt` (indexed by
+// mesh vertex): the record tile_setup_one makes from the clip-space
+// triangle, without transforming the triangle's vertices again.
 template <bool SMOOTH>
-__global__ __launch_bounds__(256) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
-                                                         TilePool P, Light lt) {
+__device__ __forceinline__ void tile_setup_shared(const pbr_scene_desc& D, const TileFrame& F,
+                                                  const TilePool& P, const Light& lt, uint32_t g,
+                                                  uint32_t slot, int s, int j, uint32_t local,
+                                                  uint32_t i, uint32_t t, const VertRec* vt) {
+  const pbr_node& nd = D.nodes[j];
+  const pbr_mesh& me = D.meshes[nd.mesh];
+  const size_t row = nd.shared ? (size_t)i : (size_t)s * (size_t)nd.instances_per_scene + i;
+  const float4 c4 = reinterpret_cast<const float4*>(nd.colors)[row];
+  const float rgba[4] = {c4.x, c4.y, c4.z, c4.w};
+  const bool flat = nd.flat != 0;
+  const int32_t* tri = D.indices + (size_t)(me.index_offset + (int)t) * 3;
+  const VertRec a = vt[tri[0]], b = vt[tri[1]], c = vt[tri[2]];
+  float flat_rgba[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (!SMOOTH || flat) shade(a.n, rgba, lt, flat_rgba);
+  const uint32_t inb = a.bits & b.bits & c.bits;
+  const bool all_in = ((inb >> 1) & 0x3fu) == 0x3fu;
+  const bool all_out = ((inb >> 7) & 0x3fu) != 0u;
+  short4 bb = make_short4(INT16_MAX, -1, INT16_MAX, -1);
+  float* sh = ((SMOOTH && !flat) || F.float_out) ? P.shades + (size_t)g * F.shw : nullptr;
+  if (all_in) {
+    const SnapVert sa = {a.X, a.Y, a.d, a.invw, (a.bits & 1u) != 0u};
+    const SnapVert sb = {b.X, b.Y, b.d, b.invw, (b.bits & 1u) != 0u};
+    const SnapVert sc = {c.X, c.Y, c.d, c.invw, (c.bits & 1u) != 0u};
+    FixTri T;
+    if (setup_snapped(sa, sb, sc, F.W, F.H, 0, F.H - 1, T)) {
+      CVert va, vb, vc;  // write_grec reads the normals only
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        va.n[q] = a.n[q];
+        vb.n[q] = b.n[q];
+        vc.n[q] = c.n[q];
+      }
+      write_grec<SMOOTH>(T, slot, flat, flat_rgba, rgba, va, vb, vc, P.recs + g, sh);
+      bb = make_short4((short)T.i0, (short)T.i1, (short)T.j0, (short)T.j1);
+    }
+  } else if (!all_out) {
+    CVert cv[3];
+    float rgba2[4];
+    bool flat2;
+    fetch_node_triangle(D, s, j, local, cv, rgba2, flat2);
+    setup_clipped<SMOOTH>(F, P, g, slot, cv, flat, flat_rgba, sh, bb);
+  }
+  P.box[g] = bb;
+  if (!F.float_out) P.col8[g] = pack_rgba8(flat_rgba);
+}
+
+constexpr int kSetupBlock = 256;
+constexpr int kWaveVerts = 256;  // vertex records per wave (8 KiB of LDS)
+
+// Each wave takes a contiguous range of pool slots and walks it in chunks
+// that stay within one (scene, node) and end on an instance boundary: the
+// chunk's instances' vertices are transformed and snapped once into the
+// wave's LDS table (a UV sphere's 224 triangles share 114 vertices, a
+// cylinder's 128 share 130, ten boxes' 120 share 240), then every lane sets
+// up triangles of the chunk from three table entries.  Only the wave
+// synchronises, so waves overlap each other's loads freely.  Meshes with
+// more vertices than the table set each triangle up from its own
+// clip-space vertices.
+template <bool SMOOTH>
+__global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
+                                                                 TilePool P, Light lt) {
+  __shared__ VertRec vts[kSetupBlock / 64][kWaveVerts];
+  const int tid = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  VertRec* vt = vts[wave];
   const uint32_t total = pool_total(D, F, P);
-  // each workgroup takes a contiguous range of pool slots, 256 consecutive
-  // slots per step (coalesced stores), so a thread's next slot is usually in
-  // the same scene as its last
-  const uint64_t per = ((((uint64_t)total + gridDim.x - 1) / gridDim.x) + 255u) & ~(uint64_t)255u;
-  const uint64_t g_end = min((uint64_t)total, (uint64_t)(blockIdx.x + 1) * per);
+  const uint64_t nw = (uint64_t)gridDim.x * (kSetupBlock / 64);
+  const uint64_t per = ((((uint64_t)total + nw - 1) / nw) + 63u) & ~(uint64_t)63u;
+  const uint64_t wid = (uint64_t)blockIdx.x * (kSetupBlock / 64) + (uint64_t)wave;
+  uint64_t g = wid * per;
+  const uint64_t g_end = min((uint64_t)total, (wid + 1) * per);
   int s = -1;
-  for (uint64_t g64 = (uint64_t)blockIdx.x * per + threadIdx.x; g64 < g_end; g64 += 256u) {
-    const uint32_t g = (uint32_t)g64;
+  while (g < g_end) {
+    // the chunk's first slot (the same on every lane)
     uint32_t slot;
-    pool_locate(D, F, P, g, s, slot);
+    pool_locate(D, F, P, (uint32_t)g, s, slot);
     int j;
     uint32_t local;
     locate_node(D, s, slot, j, local);
-    CVert cv[3];
-    float rgba[4];
-    bool flat;
-    fetch_node_triangle(D, s, j, local, cv, rgba, flat);
-    tile_setup_one<SMOOTH>(F, P, lt, g, slot, cv, rgba, flat);
+    const pbr_node& nd = D.nodes[j];
+    const pbr_mesh& me = D.meshes[nd.mesh];
+    const uint32_t ntri = (uint32_t)me.num_triangles, nv = (uint32_t)me.num_vertices;
+    int cnt = nd.instances_per_scene;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
+    const uint32_t i0 = local / ntri, t0 = local - i0 * ntri;
+    // whole instances of this (scene, node) whose vertices fit the table
+    const uint32_t ninst =
+        nv <= (uint32_t)kWaveVerts ? min((uint32_t)cnt - i0, (uint32_t)kWaveVerts / nv) : 0u;
+    const uint32_t span = ninst > 0 ? ninst * ntri - t0 : min(64u, (uint32_t)cnt * ntri - local);
+    const uint32_t n = (uint32_t)min((uint64_t)span, g_end - g);
+    if (ninst > 0) {
+      const size_t row0 =
+          nd.shared ? (size_t)i0 : (size_t)s * (size_t)nd.instances_per_scene + i0;
+      float VP[16];
+      load_mat4(D.view_proj + (size_t)s * 16, VP);
+      for (uint32_t v = (uint32_t)lane; v < ninst * nv; v += 64u) {
+        const uint32_t di = v / nv, k = v - di * nv;
+        float M[16];
+        load_mat4(nd.transforms + (row0 + di) * 16, M);
+        vertex_record(M, VP, D.vertices + (size_t)(me.vertex_offset + (int)k) * 6, F.W, F.H,
+                      F.guard, vt[v]);
+      }
+      wave_fence();
+      for (uint32_t q = (uint32_t)lane; q < n; q += 64u) {
+        const uint32_t u = t0 + q, di = u / ntri, t = u - di * ntri;
+        tile_setup_shared<SMOOTH>(D, F, P, lt, (uint32_t)g + q, slot + q, s, j, local + q,
+                                  i0 + di, t, vt + di * nv);
+      }
+      wave_fence();  // the next chunk rewrites the table
+    } else {
+      for (uint32_t q = (uint32_t)lane; q < n; q += 64u) {
+        CVert cv[3];
+        float rgba[4];
+        bool flat;
+        fetch_node_triangle(D, s, j, local + q, cv, rgba, flat);
+        tile_setup_one<SMOOTH>(F, P, lt, (uint32_t)g + q, slot + q, cv, rgba, flat);
+      }
+    }
+    g += n;
   }
 }
 
