@@ -1,18 +1,23 @@
 // pbr_tiles.hip - the tile path: any triangle count per scene, each triangle
 // set up exactly once.
 //
-//   tile_prep_kernel    per-scene triangle-slot totals (instance counts x
-//                       mesh triangles) -> exclusive prefix = each scene's
-//                       first slot in the record pool; resets the work queue
-//   tile_setup_kernel   one thread per (scene, slot): transform (basic.vert),
-//                       clip class, snap, cull, depth plane, flat colour ->
-//                       pixel bounding box + 64-byte record + colour in the
-//                       pool (triangles that straddle a clip plane keep the
-//                       union box of their sub-triangles and are re-derived
-//                       where they are drawn)
+//   tile_prep_kernel    zeroes the per-scene visible counts and the work
+//                       queue; with per-scene instance counts, the
+//                       per-scene triangle-slot totals -> exclusive prefix =
+//                       each scene's first slot in the record pool
+//   tile_setup_kernel   each wave walks instance-aligned chunks of pool
+//                       slots: the chunk's instance vertices are transformed
+//                       (basic.vert) and snapped once into an LDS table, then
+//                       one lane per slot: clip class, cull, depth plane,
+//                       flat colour -> 64-byte record (+ shade data) in the
+//                       pool, and the (pixel box, slot) of every triangle
+//                       with fragments appended to its scene's visible list
+//                       (wave ballot, one atomic per 64 slots; triangles
+//                       that straddle a clip plane keep the union box of
+//                       their sub-triangles and are re-derived where drawn)
 //   tile_raster_kernel  one workgroup per (scene, square tile), optionally
-//                       persistent over a work queue: the scene's boxes are
-//                       tested 256 at a time, survivors are compacted with a
+//                       persistent over a work queue: the scene's visible
+//                       boxes are tested 1024 at a time, survivors are compacted with a
 //                       wave ballot into two LDS queues, small ones are
 //                       rasterised by one lane each and large ones by a whole
 //                       wave, and every covered pixel takes the LDS atomicMin
@@ -66,10 +71,13 @@ struct TileFrame {
 struct TilePool {
   uint32_t* hdr;        // [0] set when the slots exceed the budget, [1] queue head
   uint32_t* scene_off;  // [N + 1] first pool slot of each scene (counts only)
-  short4* box;          // [budget] pixel box i0, i1, j0, j1 (GL rows); dead: i0 > i1
-  GRec* recs;           // [budget]
-  uint32_t* col8;       // [budget] packed RGBA8 of flat records
-  float* shades;        // [budget][shw]
+  uint32_t* vcount;     // [N] slots of each scene that produce fragments
+  short4* box;          // [budget] per scene, from its first pool slot: pixel
+                        // boxes i0, i1, j0, j1 (GL rows) of those slots ...
+  uint32_t* vslot;      // [budget] ... and their scene slots, in no set order
+  GRec* recs;           // [budget] by pool slot
+  uint32_t* col8;       // [budget] packed RGBA8 of flat records, by pool slot
+  float* shades;        // [budget][shw], by pool slot
 };
 
 __device__ __forceinline__ void scene_range(const TileFrame& F, const TilePool& P, int s,
@@ -85,50 +93,66 @@ __device__ __forceinline__ void scene_range(const TileFrame& F, const TilePool& 
   }
 }
 
+// First pool slot of scene s.
+__device__ __forceinline__ uint32_t scene_off_of(const TileFrame& F, const TilePool& P, int s) {
+  return F.uniform ? (uint32_t)min((uint64_t)s * F.max_slots, (uint64_t)F.budget)
+                   : min(P.scene_off[s], F.budget);
+}
+
+// Slot triangles of scene s: its nodes' active instances x mesh triangles.
+__device__ __forceinline__ uint32_t scene_slots(const pbr_scene_desc& D, int s) {
+  uint32_t tot = 0;
+  for (int j = 0; j < D.num_nodes; ++j) {
+    const pbr_node& nd = D.nodes[j];
+    int cnt = nd.instances_per_scene;
+    if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
+    tot += (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
+  }
+  return tot;
+}
+
+// One workgroup: zeroes the per-scene visible counts and the queue head and,
+// with per-scene instance counts, writes every scene's first pool slot.
+// Each thread takes a contiguous run of scenes: it sums their slots, one
+// workgroup scan gives the run's first slot, and the run is walked again
+// to write the offsets.
 __global__ __launch_bounds__(1024) void tile_prep_kernel(pbr_scene_desc D, TileFrame F,
                                                          TilePool P, int compute_offsets) {
-  __shared__ uint32_t part[1024];
-  __shared__ unsigned long long carry;
-  const int tid = (int)threadIdx.x;
+  __shared__ unsigned long long part[1024 / 64];
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) {
-    carry = 0ull;
     P.hdr[0] = 0u;
     P.hdr[1] = 0u;
   }
-  __syncthreads();
+  for (int s = tid; s < D.num_scenes; s += 1024) P.vcount[s] = 0u;
   if (!compute_offsets) return;
-  for (int base = 0; base < D.num_scenes; base += 1024) {
-    const int s = base + tid;
-    uint32_t tot = 0;
-    if (s < D.num_scenes) {
-      for (int j = 0; j < D.num_nodes; ++j) {
-        const pbr_node& nd = D.nodes[j];
-        int cnt = nd.instances_per_scene;
-        if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
-        tot += (uint32_t)cnt * (uint32_t)D.meshes[nd.mesh].num_triangles;
-      }
-    }
-    part[tid] = tot;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-      const uint32_t t = tid >= d ? part[tid - d] : 0u;
-      __syncthreads();
-      part[tid] += t;
-      __syncthreads();
-    }
-    const unsigned long long c = carry;
-    if (s < D.num_scenes) {
-      const unsigned long long o = c + part[tid] - tot;
-      P.scene_off[s] = o > 0xffffffffull ? 0xffffffffu : (uint32_t)o;
-    }
-    __syncthreads();
-    if (tid == 1023) carry = c + part[1023];
-    __syncthreads();
+  const int run = (D.num_scenes + 1023) / 1024;
+  const int s0 = min(D.num_scenes, tid * run), s1 = min(D.num_scenes, s0 + run);
+  unsigned long long mine = 0;
+  for (int s = s0; s < s1; ++s) mine += scene_slots(D, s);
+  unsigned long long x = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long y = __shfl_up(x, (unsigned)d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) part[wave] = x;
+  __syncthreads();
+  unsigned long long before = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < 1024 / 64; ++w) {
+    const unsigned long long pw = part[w];
+    before += w < wave ? pw : 0ull;
+    total += pw;
+  }
+  unsigned long long o = before + x - mine;
+  for (int s = s0; s < s1; ++s) {
+    P.scene_off[s] = o > 0xffffffffull ? 0xffffffffu : (uint32_t)o;
+    o += scene_slots(D, s);
   }
   if (tid == 0) {
-    const unsigned long long c = carry;
-    P.scene_off[D.num_scenes] = c > 0xffffffffull ? 0xffffffffu : (uint32_t)c;
-    if (c > (unsigned long long)F.budget) P.hdr[0] = 1u;
+    P.scene_off[D.num_scenes] = total > 0xffffffffull ? 0xffffffffu : (uint32_t)total;
+    if (total > (unsigned long long)F.budget) P.hdr[0] = 1u;
   }
 }
 
@@ -235,9 +259,10 @@ __device__ __forceinline__ void setup_clipped(const TileFrame& F, const TilePool
         make_float4(flat_rgba[0], flat_rgba[1], flat_rgba[2], flat_rgba[3]);
 }
 
-// Setup of pool slot g (scene slot `slot`): box, record, colours.
+// Setup of pool slot g (scene slot `slot`): record and colours; returns the
+// pixel box (i0 > i1: no fragment).
 template <bool SMOOTH>
-__device__ __forceinline__ void tile_setup_one(const TileFrame& F, const TilePool& P,
+__device__ __forceinline__ short4 tile_setup_one(const TileFrame& F, const TilePool& P,
                                                const Light& lt, uint32_t g, uint32_t slot,
                                                const CVert cv[3], const float rgba[4],
                                                bool flat) {
@@ -258,15 +283,16 @@ __device__ __forceinline__ void tile_setup_one(const TileFrame& F, const TilePoo
   } else if (!all_out) {
     setup_clipped<SMOOTH>(F, P, g, slot, cv, flat, flat_rgba, sh, bb);
   }
-  P.box[g] = bb;
-  if (!F.float_out) P.col8[g] = pack_rgba8(flat_rgba);
+  // the packed colour is read only for flat winners of a uint8 render
+  if (!F.float_out && (!SMOOTH || flat) && bb.x <= bb.y) P.col8[g] = pack_rgba8(flat_rgba);
+  return bb;
 }
 
 // The setup of slot g from its instance's vertex records `vt` (indexed by
 // mesh vertex): the record tile_setup_one makes from the clip-space
 // triangle, without transforming the triangle's vertices again.
 template <bool SMOOTH>
-__device__ __forceinline__ void tile_setup_shared(const pbr_scene_desc& D, const TileFrame& F,
+__device__ __forceinline__ short4 tile_setup_shared(const pbr_scene_desc& D, const TileFrame& F,
                                                   const TilePool& P, const Light& lt, uint32_t g,
                                                   uint32_t slot, int s, int j, uint32_t local,
                                                   uint32_t i, uint32_t t, const VertRec* vt) {
@@ -308,8 +334,26 @@ __device__ __forceinline__ void tile_setup_shared(const pbr_scene_desc& D, const
     fetch_node_triangle(D, s, j, local, cv, rgba2, flat2);
     setup_clipped<SMOOTH>(F, P, g, slot, cv, flat, flat_rgba, sh, bb);
   }
-  P.box[g] = bb;
-  if (!F.float_out) P.col8[g] = pack_rgba8(flat_rgba);
+  if (!F.float_out && (!SMOOTH || flat) && bb.x <= bb.y) P.col8[g] = pack_rgba8(flat_rgba);
+  return bb;
+}
+
+// Appends the lanes' visible (box, slot) pairs to scene s's list (one LDS-
+// free wave ballot and one atomic per call).
+__device__ __forceinline__ void append_visible(const TilePool& P, uint32_t off, int s, bool vis,
+                                               short4 bb, uint32_t slot) {
+  const uint64_t m = __ballot(vis);
+  if (!m) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int first = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(&P.vcount[s], (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, first);
+  if (vis) {
+    const uint32_t k = off + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    P.box[k] = bb;
+    P.vslot[k] = slot;
+  }
 }
 
 constexpr int kSetupBlock = 256;
@@ -369,19 +413,29 @@ __global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc 
                       F.guard, vt[v]);
       }
       wave_fence();
-      for (uint32_t q = (uint32_t)lane; q < n; q += 64u) {
-        const uint32_t u = t0 + q, di = u / ntri, t = u - di * ntri;
-        tile_setup_shared<SMOOTH>(D, F, P, lt, (uint32_t)g + q, slot + q, s, j, local + q,
-                                  i0 + di, t, vt + di * nv);
+      for (uint32_t q0 = 0; q0 < n; q0 += 64u) {
+        const uint32_t q = q0 + (uint32_t)lane;
+        short4 bb = make_short4(INT16_MAX, -1, INT16_MAX, -1);
+        if (q < n) {
+          const uint32_t u = t0 + q, di = u / ntri, t = u - di * ntri;
+          bb = tile_setup_shared<SMOOTH>(D, F, P, lt, (uint32_t)g + q, slot + q, s, j,
+                                         local + q, i0 + di, t, vt + di * nv);
+        }
+        append_visible(P, scene_off_of(F, P, s), s, bb.x <= bb.y, bb, slot + q);
       }
       wave_fence();  // the next chunk rewrites the table
     } else {
-      for (uint32_t q = (uint32_t)lane; q < n; q += 64u) {
-        CVert cv[3];
-        float rgba[4];
-        bool flat;
-        fetch_node_triangle(D, s, j, local + q, cv, rgba, flat);
-        tile_setup_one<SMOOTH>(F, P, lt, (uint32_t)g + q, slot + q, cv, rgba, flat);
+      for (uint32_t q0 = 0; q0 < n; q0 += 64u) {
+        const uint32_t q = q0 + (uint32_t)lane;
+        short4 bb = make_short4(INT16_MAX, -1, INT16_MAX, -1);
+        if (q < n) {
+          CVert cv[3];
+          float rgba[4];
+          bool flat;
+          fetch_node_triangle(D, s, j, local + q, cv, rgba, flat);
+          bb = tile_setup_one<SMOOTH>(F, P, lt, (uint32_t)g + q, slot + q, cv, rgba, flat);
+        }
+        append_visible(P, scene_off_of(F, P, s), s, bb.x <= bb.y, bb, slot + q);
       }
     }
     g += n;
@@ -588,6 +642,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void tile_raster_kernel(pbr_scene_desc D
     t.tj0 = t.tj1 - (t.h - 1);
     uint32_t n;
     scene_range(F, P, t.s, t.off, n);
+    n = min(n, P.vcount[t.s]);  // the scene's slots with fragments, listed first
     for (int p = tid; p < tpx; p += kTileBlock) pix[p] = (unsigned long long)kDepthOne << 32;
     if (tid == 0) {
       ns = 0u;
@@ -602,10 +657,12 @@ __attribute__((amdgpu_waves_per_eu(4))) void tile_raster_kernel(pbr_scene_desc D
       const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
       for (int k = 0; k < kPerThread; ++k) {
-        const uint32_t slot = r0 + (uint32_t)(k * kTileBlock + tid);
+        const uint32_t idx = r0 + (uint32_t)(k * kTileBlock + tid);
         bool small = false, big = false;
-        if (slot < n) {
-          const short4 b = P.box[t.off + slot];
+        uint32_t slot = 0;
+        if (idx < n) {
+          const short4 b = P.box[t.off + idx];
+          slot = P.vslot[t.off + idx];
           const int a0 = max((int)b.x, t.ti0), a1 = min((int)b.y, t.ti1);
           const int c0 = max((int)b.z, t.tj0), c1 = min((int)b.w, t.tj1);
           if (a0 <= a1 && c0 <= c1) {
@@ -846,7 +903,7 @@ struct TilePlan {
   TileFrame F;
   bool counts;
   bool smooth;
-  size_t off_scene, off_box, off_recs, off_col8, off_shades, bytes;
+  size_t off_scene, off_vcount, off_box, off_vslot, off_recs, off_col8, off_shades, bytes;
   size_t lds;  // dynamic LDS of the raster kernel
 };
 
@@ -898,8 +955,12 @@ int tiles_plan(const pbr_scene_desc* D, const pbr_render_opts* opts, TilePlan& T
   size_t o = 256;  // header
   T.off_scene = o;
   o = align256(o + 4 * ((size_t)D->num_scenes + 1));
+  T.off_vcount = o;
+  o = align256(o + 4 * (size_t)D->num_scenes);
   T.off_box = o;
   o = align256(o + 8 * (size_t)budget);
+  T.off_vslot = o;
+  o = align256(o + 4 * (size_t)budget);
   T.off_recs = o;
   o = align256(o + sizeof(GRec) * (size_t)budget);
   T.off_col8 = o;
@@ -950,7 +1011,9 @@ int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rend
   TilePool P;
   P.hdr = reinterpret_cast<uint32_t*>(ws);
   P.scene_off = reinterpret_cast<uint32_t*>(ws + T.off_scene);
+  P.vcount = reinterpret_cast<uint32_t*>(ws + T.off_vcount);
   P.box = reinterpret_cast<short4*>(ws + T.off_box);
+  P.vslot = reinterpret_cast<uint32_t*>(ws + T.off_vslot);
   P.recs = reinterpret_cast<GRec*>(ws + T.off_recs);
   P.col8 = reinterpret_cast<uint32_t*>(ws + T.off_col8);
   P.shades = reinterpret_cast<float*>(ws + T.off_shades);
@@ -959,11 +1022,11 @@ int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rend
   const int persistent = (opts && opts->persistent) ? 1 : 0;
   const Light lt = host_light(D);
   hipError_t e = hipSuccess;
-  if (T.counts || persistent) {
-    hipLaunchKernelGGL(tile_prep_kernel, dim3(1), dim3(1024), 0, st, *D, F, P, T.counts ? 1 : 0);
-    e = hipGetLastError();
-    if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile prep launch failed: %s", hipGetErrorString(e));
-  }
+  // prep: zero the per-scene visible counts (and the queue head), and with
+  // per-scene instance counts the scenes' first pool slots
+  hipLaunchKernelGGL(tile_prep_kernel, dim3(1), dim3(1024), 0, st, *D, F, P, T.counts ? 1 : 0);
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile prep launch failed: %s", hipGetErrorString(e));
   // grid-stride setup: enough blocks to fill the chip, never more than the pool
   const uint64_t need = ((uint64_t)F.budget + 255) / 256;
   const uint64_t fill = (uint64_t)device_cus() * 16;

@@ -93,15 +93,17 @@ def test_workspace_size():
     o.persistent = 1
     o.kernel = _abi.PBR_KERNEL_CHUNKED
     assert _native.workspace_bytes(d, o) == 256  # the queue head
-    # tile path: header, scene offsets, then per slot box + record + colour + 4 shade floats
+    # tile path: header, scene offsets, per-scene visible counts, then per
+    # slot box + visible slot id + record + colour + 4 shade floats
     o.kernel = _abi.PBR_KERNEL_TILES
     d.num_nodes, d.num_meshes = 1, 1
     d.meshes[0].num_triangles = 100
     d.nodes[0].instances_per_scene = 2
     d.nodes[0].flat = 1  # flat meshes keep 4 shade floats per record (smooth: 16)
     budget = 3 * 200
-    want = 256 + 256 + sum((b + 255) // 256 * 256 for b in (8 * budget, 64 * budget,
-                                                           4 * budget, 16 * budget))
+    want = 256 + 256 + 256 + sum((b + 255) // 256 * 256 for b in (8 * budget, 4 * budget,
+                                                                 64 * budget, 4 * budget,
+                                                                 16 * budget))
     assert _native.workspace_bytes(d, o) == want
     o.slot_budget = 50
     assert _native.workspace_bytes(d, o) < want
