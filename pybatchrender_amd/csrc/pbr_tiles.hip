@@ -363,10 +363,10 @@ constexpr int kWaveVerts = 256;  // vertex records per wave (8 KiB of LDS)
 // that stay within one (scene, node) and end on an instance boundary: the
 // chunk's instances' vertices are transformed and snapped once into the
 // wave's LDS table (a UV sphere's 224 triangles share 114 vertices, a
-// cylinder's 128 share 130, ten boxes' 120 share 240), then every lane sets
-// up triangles of the chunk from three table entries.  Only the wave
-// synchronises, so waves overlap each other's loads freely.  Meshes with
-// more vertices than the table set each triangle up from its own
+// cylinder's 128 share 130), then every lane sets up triangles of the chunk
+// from three table entries.  Only the wave synchronises, so waves overlap
+// each other's loads freely.  Meshes with more vertices than the table, or
+// more than 1.5 vertices per triangle, set each triangle up from its own
 // clip-space vertices.
 template <bool SMOOTH>
 __global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
@@ -395,9 +395,11 @@ __global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc 
     int cnt = nd.instances_per_scene;
     if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
     const uint32_t i0 = local / ntri, t0 = local - i0 * ntri;
-    // whole instances of this (scene, node) whose vertices fit the table
-    const uint32_t ninst =
-        nv <= (uint32_t)kWaveVerts ? min((uint32_t)cnt - i0, (uint32_t)kWaveVerts / nv) : 0u;
+    // whole instances of this (scene, node) whose vertices fit the table,
+    // for meshes whose triangles share vertices (at most 1.5 vertices per
+    // triangle: a faceted box has 2, and sets each triangle up alone)
+    const bool share = nv <= (uint32_t)kWaveVerts && 2u * nv <= 3u * ntri;
+    const uint32_t ninst = share ? min((uint32_t)cnt - i0, (uint32_t)kWaveVerts / nv) : 0u;
     const uint32_t span = ninst > 0 ? ninst * ntri - t0 : min(64u, (uint32_t)cnt * ntri - local);
     const uint32_t n = (uint32_t)min((uint64_t)span, g_end - g);
     if (ninst > 0) {
