@@ -5,26 +5,30 @@
 //                       queue; with per-scene instance counts, the
 //                       per-scene triangle-slot totals -> exclusive prefix =
 //                       each scene's first slot in the record pool
-//   tile_setup_kernel   each wave walks instance-aligned chunks of pool
-//                       slots: the chunk's instance vertices are transformed
-//                       (basic.vert) and snapped once into an LDS table, then
-//                       one lane per slot: clip class, cull, depth plane,
-//                       flat colour -> 64-byte record (+ shade data) in the
-//                       pool, and the (pixel box, slot) of every triangle
-//                       with fragments appended to its scene's visible list
-//                       (wave ballot, one atomic per 64 slots; triangles
-//                       that straddle a clip plane keep the union box of
-//                       their sub-triangles and are re-derived where drawn)
+//   tile_setup_kernel   each wave takes an equal range of pool slots,
+//                       walked in instance-aligned chunks: the
+//                       chunk's instance vertices are transformed
+//                       (basic.vert) and snapped once into an LDS table, a
+//                       cheap test per slot (clip class, cull, pixel-centre
+//                       box) queues the live slots in LDS, then one lane per
+//                       live slot: depth plane, flat colour -> 64-byte
+//                       record (+ shade data) in the pool, and the (pixel
+//                       box, slot) of every triangle with fragments
+//                       appended to its scene's visible list (wave ballot,
+//                       one atomic per 64 slots; triangles that straddle a
+//                       clip plane keep the union box of their
+//                       sub-triangles and are re-derived where drawn)
 //   tile_raster_kernel  one workgroup per (scene, square tile), optionally
 //                       persistent over a work queue: the scene's visible
-//                       boxes are tested 1024 at a time, survivors are compacted with a
-//                       wave ballot into two LDS queues, small ones are
-//                       rasterised by one lane each and large ones by a whole
-//                       wave, and every covered pixel takes the LDS atomicMin
-//                       of (depth bits << 32 | slot << 3 | sub-triangle); the
-//                       minimum is the same winner the in-order strict-less
-//                       depth test picks.  A resolve pass shades the winners
-//                       and writes the tile into the NCHW outputs.
+//                       boxes are tested 1024 at a time, survivors are
+//                       compacted with a wave ballot into two LDS queues,
+//                       small ones are rasterised by one lane each and large
+//                       ones by a whole wave, and every covered pixel takes
+//                       the LDS atomicMin of (depth bits << 32 | slot << 3 |
+//                       sub-triangle); the minimum is the same winner the
+//                       in-order strict-less depth test picks.  A resolve
+//                       pass shades the winners and writes the tile into the
+//                       NCHW outputs.
 //
 // The arithmetic (snap, fill rule, depth plane, shading) is the shared code
 // of pbr_device.h, so the output is bit-identical to the other kernels.
@@ -295,15 +299,14 @@ template <bool SMOOTH>
 __device__ __forceinline__ short4 tile_setup_shared(const pbr_scene_desc& D, const TileFrame& F,
                                                   const TilePool& P, const Light& lt, uint32_t g,
                                                   uint32_t slot, int s, int j, uint32_t local,
-                                                  uint32_t i, uint32_t t, const VertRec* vt) {
+                                                  uint32_t i, uint32_t tri, const VertRec* vt) {
   const pbr_node& nd = D.nodes[j];
   const pbr_mesh& me = D.meshes[nd.mesh];
   const size_t row = nd.shared ? (size_t)i : (size_t)s * (size_t)nd.instances_per_scene + i;
   const float4 c4 = reinterpret_cast<const float4*>(nd.colors)[row];
   const float rgba[4] = {c4.x, c4.y, c4.z, c4.w};
   const bool flat = nd.flat != 0;
-  const int32_t* tri = D.indices + (size_t)(me.index_offset + (int)t) * 3;
-  const VertRec a = vt[tri[0]], b = vt[tri[1]], c = vt[tri[2]];
+  const VertRec a = vt[tri & 0xffu], b = vt[(tri >> 8) & 0xffu], c = vt[tri >> 16];
   float flat_rgba[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   if (!SMOOTH || flat) shade(a.n, rgba, lt, flat_rgba);
   const uint32_t inb = a.bits & b.bits & c.bits;
@@ -338,6 +341,39 @@ __device__ __forceinline__ short4 tile_setup_shared(const pbr_scene_desc& D, con
   return bb;
 }
 
+// Mesh triangle t's vertex indices packed a | b << 8 | c << 16 (meshes
+// set up through the vertex table have at most 256 vertices).
+__device__ __forceinline__ uint32_t packed_triangle(const pbr_scene_desc& D, const pbr_mesh& me,
+                                                    uint32_t t) {
+  const int32_t* tri = D.indices + (size_t)(me.index_offset + (int)t) * 3;
+  return (uint32_t)tri[0] | ((uint32_t)tri[1] << 8) | ((uint32_t)tri[2] << 16);
+}
+
+// Whether triangle `tri` (vertex indices a | b << 8 | c << 16) of an
+// instance whose vertex records are `vt` can produce fragments: the tests that end setup_snapped early (all three
+// vertices outside one clip plane, a vertex that did not snap, area <= 0,
+// no pixel centre inside the bounding box), taken on the records alone.  A
+// triangle that straddles a clip plane is live: its clipped setup decides.
+__device__ __forceinline__ bool shared_slot_live(const TileFrame& F, uint32_t tri,
+                                                 const VertRec* vt) {
+  const VertRec& a = vt[tri & 0xffu];
+  const VertRec& b = vt[(tri >> 8) & 0xffu];
+  const VertRec& c = vt[tri >> 16];
+  const uint32_t inb = a.bits & b.bits & c.bits;
+  if ((inb >> 7) & 0x3fu) return false;
+  if (((inb >> 1) & 0x3fu) != 0x3fu) return true;
+  if (!(inb & 1u)) return false;
+  const int32_t X[3] = {a.X, b.X, c.X}, Y[3] = {a.Y, b.Y, c.Y};
+  const double area = (double)(X[1] - X[0]) * (double)(Y[2] - Y[0]) -
+                      (double)(X[2] - X[0]) * (double)(Y[1] - Y[0]);
+  if (!(area > 0.0)) return false;
+  const int32_t xmin = min(X[0], min(X[1], X[2])), xmax = max(X[0], max(X[1], X[2]));
+  const int32_t ymin = min(Y[0], min(Y[1], Y[2])), ymax = max(Y[0], max(Y[1], Y[2]));
+  const int32_t i0 = max((xmin - 128 + 255) >> 8, 0), i1 = min((xmax - 128) >> 8, F.W - 1);
+  const int32_t j0 = max((ymin - 128 + 255) >> 8, 0), j1 = min((ymax - 128) >> 8, F.H - 1);
+  return i0 <= i1 && j0 <= j1;
+}
+
 // Appends the lanes' visible (box, slot) pairs to scene s's list (one LDS-
 // free wave ballot and one atomic per call).
 __device__ __forceinline__ void append_visible(const TilePool& P, uint32_t off, int s, bool vis,
@@ -356,32 +392,32 @@ __device__ __forceinline__ void append_visible(const TilePool& P, uint32_t off, 
   }
 }
 
-constexpr int kSetupBlock = 256;
-constexpr int kWaveVerts = 256;  // vertex records per wave (8 KiB of LDS)
+// One wave per workgroup: a wave that finishes its range frees its CU slot
+// at once (four-wave workgroups held it until their slowest wave ended; on
+// MI355X for the 8192-scene uneven batch, 870 -> 790 us).
+constexpr int kSetupBlock = 64;
+constexpr int kWaveVerts = 228;  // vertex records per wave: two 114-vertex spheres (7.1 KiB)
 
-// Each wave takes a contiguous range of pool slots and walks it in chunks
-// that stay within one (scene, node) and end on an instance boundary: the
-// chunk's instances' vertices are transformed and snapped once into the
-// wave's LDS table (a UV sphere's 224 triangles share 114 vertices, a
-// cylinder's 128 share 130), then every lane sets up triangles of the chunk
-// from three table entries.  Only the wave synchronises, so waves overlap
-// each other's loads freely.  Meshes with more vertices than the table, or
-// more than 1.5 vertices per triangle, set each triangle up from its own
-// clip-space vertices.
+// Meshes whose instances are set up through the wave's vertex table: at
+// most kWaveVerts vertices, and at most 1.5 vertices per triangle (a faceted
+// box has 2, and sets each triangle up alone).
+__device__ __forceinline__ bool mesh_shares(const pbr_mesh& me) {
+  return me.num_vertices <= kWaveVerts && 2 * me.num_vertices <= 3 * me.num_triangles;
+}
+
+// The wave's setup of pool slots [g, g_end): chunks that stay within one
+// (scene, node) and end on an instance boundary.  The chunk's instances'
+// vertices are transformed and snapped once into the wave's LDS table `vt`
+// (a UV sphere's 224 triangles share 114 vertices, a cylinder's 128 share
+// 130), then lanes set up triangles of the chunk from three table entries.
+// Only the wave synchronises, so waves overlap each other's loads freely.
+// Other meshes (mesh_shares) set each triangle up from its own clip-space
+// vertices.  `s` is the scene of slot g (or any smaller one; -1: unknown).
 template <bool SMOOTH>
-__global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
-                                                                 TilePool P, Light lt) {
-  __shared__ VertRec vts[kSetupBlock / 64][kWaveVerts];
-  const int tid = (int)threadIdx.x;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-  VertRec* vt = vts[wave];
-  const uint32_t total = pool_total(D, F, P);
-  const uint64_t nw = (uint64_t)gridDim.x * (kSetupBlock / 64);
-  const uint64_t per = ((((uint64_t)total + nw - 1) / nw) + 63u) & ~(uint64_t)63u;
-  const uint64_t wid = (uint64_t)blockIdx.x * (kSetupBlock / 64) + (uint64_t)wave;
-  uint64_t g = wid * per;
-  const uint64_t g_end = min((uint64_t)total, (wid + 1) * per);
-  int s = -1;
+__device__ __forceinline__ void setup_range(const pbr_scene_desc& D, const TileFrame& F,
+                                            const TilePool& P, const Light& lt, int lane,
+                                            VertRec* vt, uint16_t* qu, uint64_t g,
+                                            uint64_t g_end, int s) {
   while (g < g_end) {
     // the chunk's first slot (the same on every lane)
     uint32_t slot;
@@ -395,12 +431,11 @@ __global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc 
     int cnt = nd.instances_per_scene;
     if (nd.counts) cnt = min(max(nd.counts[s], 0), nd.instances_per_scene);
     const uint32_t i0 = local / ntri, t0 = local - i0 * ntri;
-    // whole instances of this (scene, node) whose vertices fit the table,
-    // for meshes whose triangles share vertices (at most 1.5 vertices per
-    // triangle: a faceted box has 2, and sets each triangle up alone)
-    const bool share = nv <= (uint32_t)kWaveVerts && 2u * nv <= 3u * ntri;
-    const uint32_t ninst = share ? min((uint32_t)cnt - i0, (uint32_t)kWaveVerts / nv) : 0u;
-    const uint32_t span = ninst > 0 ? ninst * ntri - t0 : min(64u, (uint32_t)cnt * ntri - local);
+    // whole instances of this (scene, node) whose vertices fit the table;
+    // at most 2^16 slots per chunk (the queue holds 16-bit positions)
+    const uint32_t ninst = mesh_shares(me) ? min((uint32_t)cnt - i0, (uint32_t)kWaveVerts / nv) : 0u;
+    const uint32_t span =
+        ninst > 0 ? min(ninst * ntri - t0, 65536u) : min(64u, (uint32_t)cnt * ntri - local);
     const uint32_t n = (uint32_t)min((uint64_t)span, g_end - g);
     if (ninst > 0) {
       const size_t row0 =
@@ -415,15 +450,44 @@ __global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc 
                       F.guard, vt[v]);
       }
       wave_fence();
-      for (uint32_t q0 = 0; q0 < n; q0 += 64u) {
-        const uint32_t q = q0 + (uint32_t)lane;
+      // most slots produce no fragment (back faces, triangles between pixel
+      // centres): a cheap test per slot queues the live ones in LDS, and the
+      // full setup runs on 64 queued slots at a time
+      uint32_t pend = 0;  // queued slots (uniform)
+      const uint32_t soff = scene_off_of(F, P, s);
+      for (uint32_t q0 = 0; q0 < n || pend > 0;) {
+        if (q0 < n && pend < 64u) {
+          const uint32_t q = q0 + (uint32_t)lane;
+          bool live = false;
+          if (q < n) {
+            const uint32_t u = t0 + q, di = u / ntri, t = u - di * ntri;
+            live = shared_slot_live(F, packed_triangle(D, me, t), vt + di * nv);
+          }
+          const uint64_t m = __ballot(live);
+          if (live) qu[pend + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)q;
+          pend += (uint32_t)__popcll(m);
+          q0 += 64u;
+          wave_fence();
+          if (pend < 64u && q0 < n) continue;
+        }
+        // full setup of up to 64 queued slots, then the rest moves down
+        const uint32_t take = min(pend, 64u);
         short4 bb = make_short4(INT16_MAX, -1, INT16_MAX, -1);
-        if (q < n) {
+        uint32_t q = 0;
+        if ((uint32_t)lane < take) {
+          q = qu[lane];
           const uint32_t u = t0 + q, di = u / ntri, t = u - di * ntri;
           bb = tile_setup_shared<SMOOTH>(D, F, P, lt, (uint32_t)g + q, slot + q, s, j,
-                                         local + q, i0 + di, t, vt + di * nv);
+                                         local + q, i0 + di,
+                                         packed_triangle(D, me, t), vt + di * nv);
         }
-        append_visible(P, scene_off_of(F, P, s), s, bb.x <= bb.y, bb, slot + q);
+        append_visible(P, soff, s, bb.x <= bb.y, bb, slot + q);
+        uint16_t rest = 0;
+        if ((uint32_t)lane + 64u < pend) rest = qu[lane + 64];
+        wave_fence();
+        if ((uint32_t)lane + 64u < pend) qu[lane] = rest;
+        pend -= take;
+        wave_fence();
       }
       wave_fence();  // the next chunk rewrites the table
     } else {
@@ -442,6 +506,27 @@ __global__ __launch_bounds__(kSetupBlock) void tile_setup_kernel(pbr_scene_desc 
     }
     g += n;
   }
+}
+
+// Each wave takes an equal, contiguous range of pool slots (a multiple of
+// 64); the grid is several rounds of resident waves, so a wave whose range
+// costs less is followed by another.
+template <bool SMOOTH>
+__global__ __launch_bounds__(kSetupBlock)
+__attribute__((amdgpu_waves_per_eu(5))) void tile_setup_kernel(pbr_scene_desc D, TileFrame F,
+                                                                 TilePool P, Light lt) {
+  __shared__ VertRec vts[kSetupBlock / 64][kWaveVerts];
+  __shared__ uint16_t cq[kSetupBlock / 64][128];  // per wave: queued live slots
+  const int tid = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const uint32_t total = pool_total(D, F, P);
+  const uint64_t nw = (uint64_t)gridDim.x * (kSetupBlock / 64);
+  const uint64_t per = ((((uint64_t)total + nw - 1) / nw) + 63u) & ~(uint64_t)63u;
+  const uint64_t wid = (uint64_t)blockIdx.x * (kSetupBlock / 64) + (uint64_t)wave;
+  const uint64_t g = wid * per;
+  if (g >= (uint64_t)total) return;
+  setup_range<SMOOTH>(D, F, P, lt, lane, vts[wave], cq[wave], g,
+                      min((uint64_t)total, g + per), -1);
 }
 
 struct TileCtx {
@@ -1029,15 +1114,19 @@ int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rend
   hipLaunchKernelGGL(tile_prep_kernel, dim3(1), dim3(1024), 0, st, *D, F, P, T.counts ? 1 : 0);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile prep launch failed: %s", hipGetErrorString(e));
-  // grid-stride setup: enough blocks to fill the chip, never more than the pool
-  const uint64_t need = ((uint64_t)F.budget + 255) / 256;
-  const uint64_t fill = (uint64_t)device_cus() * 16;
+  // setup: 64 waves per CU, about three rounds of resident ones (measured
+  // on MI355X for the 8192-scene uneven batch: one round of equal ranges
+  // 1123 us, three 900 us), never more than the pool needs
+  const uint64_t need = ((uint64_t)F.budget + kSetupBlock - 1) / kSetupBlock;
+  const uint64_t fill = (uint64_t)device_cus() * 64 * 64 / kSetupBlock;
   const uint32_t sblocks = (uint32_t)std::min(need, fill);
   if (sblocks > 0) {
     if (T.smooth)
-      hipLaunchKernelGGL(tile_setup_kernel<true>, dim3(sblocks), dim3(256), 0, st, *D, F, P, lt);
+      hipLaunchKernelGGL(tile_setup_kernel<true>, dim3(sblocks), dim3(kSetupBlock), 0, st, *D, F, P,
+                         lt);
     else
-      hipLaunchKernelGGL(tile_setup_kernel<false>, dim3(sblocks), dim3(256), 0, st, *D, F, P, lt);
+      hipLaunchKernelGGL(tile_setup_kernel<false>, dim3(sblocks), dim3(kSetupBlock), 0, st, *D, F, P,
+                         lt);
     e = hipGetLastError();
     if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "tile setup launch failed: %s", hipGetErrorString(e));
   }
