@@ -903,22 +903,34 @@ __attribute__((amdgpu_waves_per_eu(4))) void tile_raster_kernel(pbr_scene_desc D
         uint32_t packed = 0;
         bool have_packed = false;
         if (covered) {
-          bool smooth = false;
-          if (SMOOTH) smooth = (P.recs[g].flags & kFlagSmooth) != 0u;
-          if (!smooth) {
+          // the record, its shade floats and its packed colour in one round
+          // of loads; the record's flags then pick what is used (a smooth
+          // record has no packed colour, a flat one of a uint8 render no
+          // shade floats: those reads return unused stale words)
+          GRec R;
+          load_grec(P.recs + g, R);
+          float sh[kShadeFloats];
+          const float4* s4 = reinterpret_cast<const float4*>(P.shades + g * F.shw);
+#pragma unroll
+          for (int q = 0; q < kShadeFloats / 4; ++q) {
+            const float4 v4 = s4[q];
+            sh[4 * q] = v4.x;
+            sh[4 * q + 1] = v4.y;
+            sh[4 * q + 2] = v4.z;
+            sh[4 * q + 3] = v4.w;
+          }
+          const uint32_t c8 = CMODE == 0 ? P.col8[g] : 0u;
+          if (!(R.flags & kFlagSmooth)) {
             if (CMODE == 0) {
-              packed = P.col8[g];
+              packed = c8;
               have_packed = true;
             } else {
-              const float4 c4 = *reinterpret_cast<const float4*>(P.shades + g * F.shw);
-              col[0] = c4.x;
-              col[1] = c4.y;
-              col[2] = c4.z;
-              col[3] = c4.w;
+              col[0] = sh[0];
+              col[1] = sh[1];
+              col[2] = sh[2];
+              col[3] = sh[3];
             }
           } else {
-            GRec R;
-            load_grec(P.recs + g, R);
             const int jg = H - 1 - yo;
             const double px = (double)(256 * x + 128), py = (double)(256 * jg + 128);
             if (R.flags & kFlagClipped) {
@@ -945,7 +957,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void tile_raster_kernel(pbr_scene_desc D
             } else {
               double A[3], B[3], Cc[3];
               edges_from_fixed(R.X, R.Y, R.flags & 7u, A, B, Cc);
-              shade_smooth(A, B, Cc, R.flags & 7u, P.shades + g * F.shw, px, py, lt, col);
+              shade_smooth(A, B, Cc, R.flags & 7u, sh, px, py, lt, col);
             }
           }
         }
