@@ -42,9 +42,13 @@ using namespace pbr_host;
 
 namespace {
 
-constexpr int kRecs = 64;        // records (= slots) per scene
-constexpr int kBg = kRecs;       // table index of the background
-constexpr int kCap = 256;        // row-list entries per band
+// An instantiation holds RECS records (= slots) per scene (32 or 64): the
+// scene's colour table keeps the background at index RECS, and a band's row
+// list holds 4 * RECS entries.  The 32-record form needs under 20 KiB of LDS
+// and is held to 64 VGPRs, so eight workgroups share a CU and the 4096-scene
+// CartPole grid runs in one round (64-record form: four, two rounds).
+template <int RECS>
+constexpr int Cap = 4 * RECS;
 constexpr int kBandRows = 64;    // rows per band at most
 // Waves sharing a scene's rows (measured on MI355X for 64 x 64 CartPole
 // frames: 2 -> 36 us, 4 -> 51 us, 1 -> 42 us per 4096-scene call).
@@ -58,7 +62,7 @@ struct RowsFrame {
   int W, H;
   float guard;
   int seg;  // setup lanes per scene (power of two >= slots, <= 64)
-  int cap;  // row-list entries per band (8..kCap)
+  int cap;  // row-list entries per band (8..Cap<RECS>)
   int vphase;  // 1: vertices transformed once into an LDS table (they fit)
 };
 
@@ -101,23 +105,24 @@ __device__ __forceinline__ typename ColOf<OM>::T pack_col(const float c[4]) {
 }
 
 // A scene's triangle records and colour table (shared by the scene's waves).
-template <bool SMOOTH, int OM>
+template <bool SMOOTH, int OM, int RECS>
 struct RowsLds {
-  RowRec rec[kRecs];
-  typename ColOf<OM>::T col[kRecs + 1];
-  uint8_t flags[kRecs];
+  RowRec rec[RECS];
+  typename ColOf<OM>::T col[RECS + 1];
+  uint8_t flags[RECS];
   int nrec;
   int any_clipped;
-  float sh[SMOOTH ? kRecs * kShadeFloats : 1];
+  float sh[SMOOTH ? RECS * kShadeFloats : 1];
 };
 
 // One wave's row lists.
+template <int RECS>
 struct ListLds {
   unsigned long long plane[3][kBandRows];
   uint32_t rowstart[kBandRows + 1];
   union {
-    int32_t mark[kCap];  // pair -> record markers (consumed before entries are written)
-    uint4 ent[kCap];     // lo | hi << 8 | ref << 16, tz bits, zx bits, RGBA8 (uint8 output)
+    int32_t mark[Cap<RECS>];  // pair -> record markers (consumed before entries are written)
+    uint4 ent[Cap<RECS>];     // lo | hi << 8 | ref << 16, tz bits, zx bits, RGBA8 (uint8 output)
   } u;
 };
 
@@ -373,10 +378,10 @@ __device__ __noinline__ int clipped_row_entries(const pbr_scene_desc* D, RowsFra
 }
 
 // (a)+(b) of one slot of scene s into the scene's LDS tables.
-template <bool SMOOTH, int OM>
+template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void setup_slot(const pbr_scene_desc& D, const RowsFrame& F,
                                            const Light& lt, int s, uint32_t slot,
-                                           RowsLds<SMOOTH, OM>& L) {
+                                           RowsLds<SMOOTH, OM, RECS>& L) {
   uint32_t local, total;
   const int node = locate_slot(D, s, slot, local, total);
   if (slot == 0) L.nrec = (int)total;
@@ -438,7 +443,8 @@ __device__ __forceinline__ void setup_slot(const pbr_scene_desc& D, const RowsFr
 
 // The vertex table of a workgroup's two scenes lives in the (not yet used)
 // row-list space.
-constexpr int kVertCap = (int)((kRowsBlock / 64) * sizeof(ListLds) / sizeof(VertRec));
+template <int RECS>
+constexpr int VertCap = (int)((kRowsBlock / 64) * sizeof(ListLds<RECS>) / sizeof(VertRec));
 
 // (a) + per-vertex (b) of vertex slot v of scene s (draw order: node,
 // instance, mesh vertex).
@@ -478,10 +484,10 @@ __device__ __forceinline__ void vertex_slot(const pbr_scene_desc& D, const RowsF
 // (a)+(b) of one slot of scene s from the scene's vertex table `vt`: the
 // same record setup_slot makes, with the vertices read back instead of
 // transformed again.
-template <bool SMOOTH, int OM>
+template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void setup_slot_v(const pbr_scene_desc& D, const RowsFrame& F,
                                              const Light& lt, int s, uint32_t slot,
-                                             RowsLds<SMOOTH, OM>& L, const VertRec* vt) {
+                                             RowsLds<SMOOTH, OM, RECS>& L, const VertRec* vt) {
   uint32_t local, total;
   const int node = locate_slot(D, s, slot, local, total);
   if (slot == 0) L.nrec = (int)total;
@@ -590,10 +596,10 @@ __device__ __forceinline__ uint32_t scene_vertex_slots(const pbr_scene_desc& D, 
 
 // Colour of a smooth winner at pixel (x, j): per-pixel interpolation of the
 // (sub-)triangle's normals.
-template <bool SMOOTH, int OM>
+template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void smooth_colour(const pbr_scene_desc& D, const RowsFrame& F,
                                               const Light& lt, int s,
-                                              const RowsLds<SMOOTH, OM>& L, uint32_t ref, int x,
+                                              const RowsLds<SMOOTH, OM, RECS>& L, uint32_t ref, int x,
                                               int j, float col[4]) {
   const uint32_t slot = ref >> 3;
   const uint32_t fl = L.flags[slot];
@@ -619,10 +625,12 @@ __device__ __forceinline__ void smooth_colour(const pbr_scene_desc& D, const Row
 
 // Row lists of GL rows [jb0, jb0 + R) for records [r0, r1) of the scene: on
 // return rowstart[0..R] and u.ent[] hold them.
-template <bool SMOOTH, int OM>
+template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsFrame& F, int s,
-                                            const RowsLds<SMOOTH, OM>& L, ListLds& Q, int lane,
-                                            int jb0, int R, int r0, int r1, int rj0, int rj1) {
+                                            const RowsLds<SMOOTH, OM, RECS>& L, ListLds<RECS>& Q,
+                                            int lane, int jb0, int R, int r0, int r1, int rj0,
+                                            int rj1) {
+  constexpr int kCap = Cap<RECS>;
   const bool mine = lane >= r0 && lane < r1;
   const int lo_row = max(rj0, jb0), hi_row = min(rj1, jb0 + R - 1);
   const int cnt = mine ? max(0, hi_row - lo_row + 1) : 0;
@@ -735,13 +743,13 @@ struct PixState {
   uint32_t ref[PXL];  // slot << 3 | sub-triangle of the winner
 };
 
-template <int PXL>
+template <int PXL, int RECS>
 __device__ __forceinline__ void init_state(PixState<PXL>& S, uint32_t bg) {
 #pragma unroll
   for (int i = 0; i < PXL; ++i) {
     S.z[i] = 1.0f;
     S.col[i] = bg;
-    S.ref[i] = (uint32_t)kBg << 3;
+    S.ref[i] = (uint32_t)RECS << 3;  // the background's table index
   }
 }
 
@@ -768,9 +776,10 @@ __device__ __forceinline__ void test_entry(const uint4& e, bool ok, int x0, cons
 // The lane's row-list entries [n0, n0 + cnt); `emax` (uniform) is the
 // longest list of the wave's rows.  Two entries are loaded per step (the
 // second is a no-op past the end of the list), an odd last one alone.
-template <int PXL, bool OM0, bool REF>
+template <int PXL, bool OM0, bool REF, int RECS>
 __device__ __forceinline__ void walk_row(const uint4* ent, uint32_t n0, int cnt, int emax, int x0,
                                          const float* pxf, PixState<PXL>& S) {
+  constexpr int kCap = Cap<RECS>;
   int k = 0;
   for (; k + 1 < emax; k += 2) {
     const bool ok0 = k < cnt, ok1 = k + 1 < cnt;
@@ -805,10 +814,10 @@ struct SceneOut {
 };
 
 // Writes the lane's PXL pixels of GL row j (x0 .. x0 + PXL - 1, those < W).
-template <bool SMOOTH, int OM, int PXL>
+template <bool SMOOTH, int OM, int PXL, int RECS>
 __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const SceneOut& so,
                                              const RowsFrame& F, const Light& lt, int s,
-                                             const RowsLds<SMOOTH, OM>& L, int x0, int j,
+                                             const RowsLds<SMOOTH, OM, RECS>& L, int x0, int j,
                                              PixState<PXL>& S) {
   if (x0 >= F.W) return;
   const uint32_t pix0 = (uint32_t)(F.H - 1 - j) * (uint32_t)F.W + (uint32_t)x0;
@@ -821,9 +830,9 @@ __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const Scen
     } else {
       cv[i] = L.col[slot];
     }
-    if (SMOOTH && slot != (uint32_t)kBg && (L.flags[slot] & kRFlagSmooth)) {
+    if (SMOOTH && slot != (uint32_t)RECS && (L.flags[slot] & kRFlagSmooth)) {
       float c4[4];
-      smooth_colour<SMOOTH, OM>(D, F, lt, s, L, S.ref[i], x0 + i, j, c4);
+      smooth_colour<SMOOTH, OM, RECS>(D, F, lt, s, L, S.ref[i], x0 + i, j, c4);
       cv[i] = pack_col<OM>(c4);
     }
   }
@@ -910,10 +919,11 @@ __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const Scen
 }
 
 // GL rows [ja, jb) of scene s, lists in Q.
-template <bool SMOOTH, int OM, int PXL, bool REF>
+template <bool SMOOTH, int OM, int PXL, bool REF, int RECS>
 __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_frame_out& O,
                                              const RowsFrame& F, const Light& lt, int s,
-                                             const RowsLds<SMOOTH, OM>& L, ListLds& Q, int lane,
+                                             const RowsLds<SMOOTH, OM, RECS>& L, ListLds<RECS>& Q,
+                                             int lane,
                                              int ja, int jb) {
   constexpr bool OM0 = OM == 0;
   const int nrec = __builtin_amdgcn_readfirstlane(L.nrec);
@@ -946,7 +956,7 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
       R = (R + 1) >> 1;
     }
     if (E <= F.cap) {
-      build_lists<SMOOTH, OM>(D, F, s, L, Q, lane, jb0, R, 0, nrec, rj0, rj1);
+      build_lists<SMOOTH, OM, RECS>(D, F, s, L, Q, lane, jb0, R, 0, nrec, rj0, rj1);
       // per-lane constants of the row walk, made after the lists so they are
       // not held through them
       const int x0 = (lane & 15) * PXL;
@@ -954,7 +964,7 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
 #pragma unroll
       for (int i = 0; i < PXL; ++i) pxf[i] = (float)(x0 + i) + 0.5f;
       uint32_t bg = 0u;
-      if constexpr (OM0) bg = L.col[kBg];
+      if constexpr (OM0) bg = L.col[RECS];
       for (int g = 0; g < R; g += 4) {
         const int b = g + q;
         const bool row_ok = b < R;
@@ -968,9 +978,9 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
             max(max(__builtin_amdgcn_readlane(cnt, 0), __builtin_amdgcn_readlane(cnt, 16)),
                 max(__builtin_amdgcn_readlane(cnt, 32), __builtin_amdgcn_readlane(cnt, 48)));
         PixState<PXL> S;
-        init_state<PXL>(S, bg);
-        walk_row<PXL, OM0, REF>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
-        if (row_ok) resolve_lane<SMOOTH, OM, PXL>(D, so, F, lt, s, L, x0, jb0 + b, S);
+        init_state<PXL, RECS>(S, bg);
+        walk_row<PXL, OM0, REF, RECS>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
+        if (row_ok) resolve_lane<SMOOTH, OM, PXL, RECS>(D, so, F, lt, s, L, x0, jb0 + b, S);
       }
       wave_sync();
     } else {
@@ -982,9 +992,9 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
 #pragma unroll
       for (int i = 0; i < PXL; ++i) pxf[i] = (float)(x0 + i) + 0.5f;
       uint32_t bg = 0u;
-      if constexpr (OM0) bg = L.col[kBg];
+      if constexpr (OM0) bg = L.col[RECS];
       PixState<PXL> S;
-      init_state<PXL>(S, bg);
+      init_state<PXL, RECS>(S, bg);
       const int has = (lane < nrec && rj0 <= j && j <= rj1) ? rw : 0;
       const int incl = wave_scan_add(has);
       const int chunk = (incl - has) / (F.cap - 7);
@@ -994,14 +1004,14 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
         if (!m) continue;
         const int r0 = __ffsll((unsigned long long)m) - 1;
         const int r1 = 64 - __clzll((long long)m);
-        build_lists<SMOOTH, OM>(D, F, s, L, Q, lane, j, 1, r0, r1, rj0, rj1);
+        build_lists<SMOOTH, OM, RECS>(D, F, s, L, Q, lane, j, 1, r0, r1, rj0, rj1);
         const uint32_t n0 = Q.rowstart[0];
         const int cnt = q == 0 ? (int)(Q.rowstart[1] - n0) : 0;
         const int emax = __builtin_amdgcn_readlane(cnt, 0);
-        walk_row<PXL, OM0, REF>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
+        walk_row<PXL, OM0, REF, RECS>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
         wave_sync();
       }
-      if (q == 0) resolve_lane<SMOOTH, OM, PXL>(D, so, F, lt, s, L, x0, j, S);
+      if (q == 0) resolve_lane<SMOOTH, OM, PXL, RECS>(D, so, F, lt, s, L, x0, j, S);
     }
     jb0 += R;
   }
@@ -1009,11 +1019,22 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
 
 // D stays the first parameter: the out-of-line clipper reads it through the
 // kernarg segment (kernarg_scene).
-template <bool SMOOTH, int OM, int PXL, bool REF>
-__global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, pbr_frame_out O,
-                                                             RowsFrame F, Light lt, float4 bg) {
-  __shared__ RowsLds<SMOOTH, OM> lds[2];
-  __shared__ ListLds lists[kRowsBlock / 64];
+// Workgroups per CU an instantiation is built for (what its LDS and
+// registers allow): the 32-record flat form eight, or four with float32
+// colours; the 64-record form four, smooth forms with bf16 / float32 colours
+// three (their LDS) or two with 8-pixel lanes.
+template <bool SMOOTH, int OM, int PXL, int RECS>
+constexpr int rows_wgs_per_cu() {
+  if (RECS == 32) return OM == 2 ? 4 : 8;
+  if (SMOOTH && OM != 0) return PXL == 8 ? 2 : 3;
+  return 4;
+}
+
+template <bool SMOOTH, int OM, int PXL, bool REF, int RECS>
+__global__ __launch_bounds__(kRowsBlock, (rows_wgs_per_cu<SMOOTH, OM, PXL, RECS>())) void rows_kernel(
+    pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg) {
+  __shared__ RowsLds<SMOOTH, OM, RECS> lds[2];
+  __shared__ ListLds<RECS> lists[kRowsBlock / 64];
   const int tid = (int)threadIdx.x;
   // the wave index is uniform within the wave: keep it (and what derives
   // from it - scene, LDS region, output bases) in scalar registers
@@ -1022,7 +1043,7 @@ __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, p
   if (wave < 2 && lane == 0) lds[wave].any_clipped = 0;
   if (wave < 2 && lane == 1) {
     const float bgc[4] = {bg.x, bg.y, bg.z, bg.w};
-    lds[wave].col[kBg] = pack_col<OM>(bgc);
+    lds[wave].col[RECS] = pack_col<OM>(bgc);
   }
   __syncthreads();
   if (F.vphase) {
@@ -1041,7 +1062,7 @@ __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, p
     const int s = s0 + sl;
     if (sl < 2) {
       if (s < D.num_scenes)
-        setup_slot_v<SMOOTH, OM>(D, F, lt, s, slot, lds[sl], vt + (sl ? nv0 : 0u));
+        setup_slot_v<SMOOTH, OM, RECS>(D, F, lt, s, slot, lds[sl], vt + (sl ? nv0 : 0u));
       else if (slot == 0)
         lds[sl].nrec = 0;
     }
@@ -1052,7 +1073,7 @@ __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, p
     const int s = s0 + sl;
     if (sl < 2) {
       if (s < D.num_scenes)
-        setup_slot<SMOOTH, OM>(D, F, lt, s, slot, lds[sl]);
+        setup_slot<SMOOTH, OM, RECS>(D, F, lt, s, slot, lds[sl]);
       else if (slot == 0)
         lds[sl].nrec = 0;
     }
@@ -1064,20 +1085,29 @@ __global__ __launch_bounds__(kRowsBlock, 4) void rows_kernel(pbr_scene_desc D, p
   const int s = s0 + sl;
   if (s >= D.num_scenes) return;
   const int ja = (F.H * part) / kWavesPerScene, jb = (F.H * (part + 1)) / kWavesPerScene;
-  raster_scene<SMOOTH, OM, PXL, REF>(D, O, F, lt, s, lds[sl], lists[wave], lane, ja, jb);
+  raster_scene<SMOOTH, OM, PXL, REF, RECS>(D, O, F, lt, s, lds[sl], lists[wave], lane, ja, jb);
 }
 
-template <bool SMOOTH, int OM, bool REF>
+template <bool SMOOTH, int OM, bool REF, int RECS>
 hipError_t launch_rows_t(const pbr_scene_desc* D, const pbr_frame_out* O, const RowsFrame& F,
                          const Light& lt, float4 bg, hipStream_t st) {
   const int grid = (D->num_scenes + 1) / 2;
-  if (D->width <= 64)
-    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 4, REF>), dim3(grid), dim3(kRowsBlock), 0, st, *D,
-                       *O, F, lt, bg);
-  else
-    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 8, REF>), dim3(grid), dim3(kRowsBlock), 0, st, *D,
-                       *O, F, lt, bg);
+  // the 32-record form runs tiles of at most 64 columns (4-pixel lanes)
+  if (RECS == 32 || D->width <= 64)
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 4, REF, RECS>), dim3(grid), dim3(kRowsBlock), 0,
+                       st, *D, *O, F, lt, bg);
+  else if constexpr (RECS == 64)
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 8, REF, RECS>), dim3(grid), dim3(kRowsBlock), 0,
+                       st, *D, *O, F, lt, bg);
   return hipGetLastError();
+}
+
+// Flat scenes: the 32-record form when `small`, else the 64-record one.
+template <int OM, bool REF>
+hipError_t launch_rows_flat(const pbr_scene_desc* D, const pbr_frame_out* O, const RowsFrame& F,
+                            const Light& lt, float4 bg, bool small, hipStream_t st) {
+  return small ? launch_rows_t<false, OM, REF, 32>(D, O, F, lt, bg, st)
+               : launch_rows_t<false, OM, REF, 64>(D, O, F, lt, bg, st);
 }
 
 }  // namespace
@@ -1085,14 +1115,13 @@ hipError_t launch_rows_t(const pbr_scene_desc* D, const pbr_frame_out* O, const 
 namespace pbr_host {
 
 bool rows_supported(const pbr_scene_desc* D) {
-  return max_slots(D) <= (uint64_t)kRecs && D->width <= 128 && D->height <= 16384;
+  return max_slots(D) <= 64 && D->width <= 128 && D->height <= 16384;
 }
 
 int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
                 hipStream_t st) {
-  if (max_slots(D) > (uint64_t)kRecs)
-    return fail(PBR_ERR_UNSUPPORTED, "row-span kernel needs at most %d triangle slots per scene",
-                kRecs);
+  if (max_slots(D) > 64)
+    return fail(PBR_ERR_UNSUPPORTED, "row-span kernel needs at most 64 triangle slots per scene");
   if (D->width > 128 || D->height > 16384)
     return fail(PBR_ERR_UNSUPPORTED, "row-span kernel needs tiles of at most 128 x 16384 pixels");
   RowsFrame F;
@@ -1103,36 +1132,39 @@ int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rende
   F.seg = 1;
   while (F.seg < slots) F.seg <<= 1;
   const int cap = opts ? opts->row_list_cap : 0;
-  if (cap != 0 && (cap < 8 || cap > kCap))
-    return fail(PBR_ERR_INVALID, "row_list_cap %d outside [8, %d]", cap, kCap);
-  F.cap = cap ? cap : kCap;
+  if (cap != 0 && (cap < 8 || cap > Cap<64>))
+    return fail(PBR_ERR_INVALID, "row_list_cap %d outside [8, %d]", cap, Cap<64>);
+  const bool smooth = any_smooth(D);
+  // flat scenes of at most 32 slots and 64 columns whose list cap fits run
+  // the 32-record form
+  const bool small = !smooth && slots <= 32 && D->width <= 64 && cap <= Cap<32>;
+  F.cap = cap ? cap : (small ? Cap<32> : Cap<64>);
   // the vertex phase needs both scenes' vertex slots in the table
   uint64_t vslots = 0;
   for (int j = 0; j < D->num_nodes; ++j)
     vslots += (uint64_t)D->nodes[j].instances_per_scene *
               (uint64_t)D->meshes[D->nodes[j].mesh].num_vertices;
-  F.vphase = (2 * vslots <= (uint64_t)kVertCap) ? 1 : 0;
+  F.vphase = (2 * vslots <= (uint64_t)(small ? VertCap<32> : VertCap<64>)) ? 1 : 0;
   const Light lt = host_light(D);
   const float4 bg = make_float4(D->background[0], D->background[1], D->background[2],
                                 D->background[3]);
-  const bool smooth = any_smooth(D);
   hipError_t e;
   switch (O->dtype) {
     case PBR_DTYPE_U8:
       if (smooth)
-        e = launch_rows_t<true, 0, true>(D, O, F, lt, bg, st);
+        e = launch_rows_t<true, 0, true, 64>(D, O, F, lt, bg, st);
       else if (O->vis)
-        e = launch_rows_t<false, 0, true>(D, O, F, lt, bg, st);
+        e = launch_rows_flat<0, true>(D, O, F, lt, bg, small, st);
       else
-        e = launch_rows_t<false, 0, false>(D, O, F, lt, bg, st);
+        e = launch_rows_flat<0, false>(D, O, F, lt, bg, small, st);
       break;
     case PBR_DTYPE_BF16:
-      e = smooth ? launch_rows_t<true, 1, true>(D, O, F, lt, bg, st)
-                 : launch_rows_t<false, 1, true>(D, O, F, lt, bg, st);
+      e = smooth ? launch_rows_t<true, 1, true, 64>(D, O, F, lt, bg, st)
+                 : launch_rows_flat<1, true>(D, O, F, lt, bg, small, st);
       break;
     default:
-      e = smooth ? launch_rows_t<true, 2, true>(D, O, F, lt, bg, st)
-                 : launch_rows_t<false, 2, true>(D, O, F, lt, bg, st);
+      e = smooth ? launch_rows_t<true, 2, true, 64>(D, O, F, lt, bg, st)
+                 : launch_rows_flat<2, true>(D, O, F, lt, bg, small, st);
       break;
   }
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "row-span launch failed: %s", hipGetErrorString(e));
