@@ -395,6 +395,41 @@ def test_rows_kernel_short_lists(cuda, cap):
     check_parity(r, f"rows kernel list cap {cap}")
 
 
+@pytest.mark.parametrize("res,cap", [((64, 64), 0), ((64, 64), 8), ((64, 64), 16),
+                                     ((33, 17), 64), ((128, 70), 0), ((64, 64), 200)])
+@pytest.mark.parametrize("dtype", ["uint8", "bfloat16", "float32"])
+def test_rows_kernel_small_scenes(cuda, res, cap, dtype):
+    """Two boxes (24 slots): flat scenes of at most 32 slots and 64 columns
+    whose list cap is at most 128 run the 32-record form of the kernel (short
+    lists: many bands and record chunks); 128 columns or a 200-entry cap
+    take the 64-record form."""
+    r = _small_scene(20, res, 36, instances=2, kernel="rows", output_dtype=dtype,
+                     row_list_cap=cap)
+    check_parity(r, f"rows kernel 24 slots {res} cap {cap} {dtype}")
+    check_parity(r, f"rows kernel 24 slots {res} cap {cap} {dtype} colour only", depth=False,
+                 vis=False)
+
+
+@pytest.mark.parametrize("cap", [0, 8])
+def test_rows_kernel_small_scene_clipping(cuda, cap):
+    """The 32-record form with triangles through the near plane (two large
+    boxes around the camera, 24 slots)."""
+    n = 8
+    gen = torch.Generator("cpu").manual_seed(37)
+    r = PBRRenderer(PBRConfig(num_scenes=n, tile_resolution=(48, 48), device="cuda",
+                              kernel="rows", row_list_cap=cap))
+    node = r.add_node("models/box", instances_per_scene=2, model_pivot_relative_point=(0.5, 0.5, 0.5))
+    node.set_transforms(workloads.random_transforms(n * 2, gen, pos_range=1.5,
+                                                    scale_range=(3.0, 8.0)))
+    node.set_colors(torch.rand(n * 2, 4, generator=gen))
+    r.add_camera(z_near=0.3)
+    r._pbr_cam.set_positions_and_lookat(torch.zeros(3), torch.tensor([0.3, 1.0, 0.2]))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, f"rows kernel 24-slot clipping cap {cap}")
+    check_parity(r, f"rows kernel 24-slot clipping cap {cap} colour only", depth=False, vis=False)
+
+
 @pytest.mark.parametrize("dtype,channels", [("float32", 4), ("bfloat16", 3), ("uint8", 2),
                                             ("uint8", 1)])
 def test_rows_kernel_smooth_dtypes(cuda, dtype, channels):
