@@ -2,18 +2,20 @@
 // at most 128 pixel columns, setup and rasterisation in one launch, no
 // workspace.
 //
-// One workgroup of four waves takes two scenes, two waves per scene.
+// One workgroup takes two scenes: four waves, two per scene, or (flat scenes
+// of at most 32 slots and 64 columns) two waves, one per scene.
 //   setup  one lane per (scene, slot) - both scenes share the first wave
 //          when they have at most 32 slots: transform (basic.vert), clip
 //          class, snap, cull, depth plane and flat colour -> a 64-byte record
 //          in LDS indexed by slot.  A triangle that straddles a clip plane
 //          keeps only the union of its sub-triangles' rows; its
 //          sub-triangles are re-derived where they are drawn.
-//   lists  (per wave = per half of a scene's rows, per band of rows) the
+//   lists  (per wave = per share of a scene's rows, per band of rows) the
 //          (record, row) pairs of the band are dealt to lanes: a prefix sum
 //          over the records puts a marker at each record's first pair and a
-//          max-scan hands every lane its (record, row).  The lane computes the exact covered
-//          columns of the pair (pbr_span.h) and sets bit `record` of the
+//          max-scan hands every lane its (record, row).  The lane computes
+//          the exact covered columns of the pair (pbr_span.h) and sets bit
+//          `record` of the
 //          row's count planes; a popcount scan over the rows then places
 //          every entry at
 //            row start + sum_b 2^b popcount(plane_b & lower records)
@@ -42,18 +44,20 @@ using namespace pbr_host;
 
 namespace {
 
-// An instantiation holds RECS records (= slots) per scene (32 or 64): the
-// scene's colour table keeps the background at index RECS, and a band's row
-// list holds 4 * RECS entries.  The 32-record form needs under 20 KiB of LDS
-// and is held to 64 VGPRs, so eight workgroups share a CU and the 4096-scene
-// CartPole grid runs in one round (64-record form: four, two rounds).
-template <int RECS>
-constexpr int Cap = 4 * RECS;
+constexpr int kCap = 256;        // row-list entries per band
 constexpr int kBandRows = 64;    // rows per band at most
-// Waves sharing a scene's rows (measured on MI355X for 64 x 64 CartPole
-// frames: 2 -> 36 us, 4 -> 51 us, 1 -> 42 us per 4096-scene call).
-constexpr int kWavesPerScene = 2;
-constexpr int kRowsBlock = 128 * kWavesPerScene;  // two scenes per workgroup
+// An instantiation holds RECS records (= slots) per scene (32 or 64); the
+// scene's colour table keeps the background at index RECS.  The 32-record
+// form gives each scene one wave (16 KiB of LDS per two-scene workgroup, 128
+// VGPRs): eight workgroups share a CU, so the 4096-scene CartPole grid runs
+// in one round.  The 64-record form gives each scene two waves, each taking
+// half of its rows; four workgroups share a CU.  Measured on MI355X for that
+// grid: two rounds of the 64-record form 31.1 us, one round with two waves
+// per scene (64 VGPRs, spilling) 28.0 us, one wave per scene 26.1 us.
+template <int RECS>
+constexpr int WavesPerScene = RECS == 32 ? 1 : 2;
+template <int RECS>
+constexpr int RowsBlock = 128 * WavesPerScene<RECS>;  // two scenes per workgroup
 
 constexpr uint32_t kRFlagSmooth = 8u;
 constexpr uint32_t kRFlagClipped = 16u;  // bits 5-7: sub-triangle count
@@ -62,7 +66,7 @@ struct RowsFrame {
   int W, H;
   float guard;
   int seg;  // setup lanes per scene (power of two >= slots, <= 64)
-  int cap;  // row-list entries per band (8..Cap<RECS>)
+  int cap;  // row-list entries per band (8..kCap)
   int vphase;  // 1: vertices transformed once into an LDS table (they fit)
 };
 
@@ -116,13 +120,12 @@ struct RowsLds {
 };
 
 // One wave's row lists.
-template <int RECS>
 struct ListLds {
   unsigned long long plane[3][kBandRows];
   uint32_t rowstart[kBandRows + 1];
   union {
-    int32_t mark[Cap<RECS>];  // pair -> record markers (consumed before entries are written)
-    uint4 ent[Cap<RECS>];     // lo | hi << 8 | ref << 16, tz bits, zx bits, RGBA8 (uint8 output)
+    int32_t mark[kCap];  // pair -> record markers (consumed before entries are written)
+    uint4 ent[kCap];     // lo | hi << 8 | ref << 16, tz bits, zx bits, RGBA8 (uint8 output)
   } u;
 };
 
@@ -444,7 +447,7 @@ __device__ __forceinline__ void setup_slot(const pbr_scene_desc& D, const RowsFr
 // The vertex table of a workgroup's two scenes lives in the (not yet used)
 // row-list space.
 template <int RECS>
-constexpr int VertCap = (int)((kRowsBlock / 64) * sizeof(ListLds<RECS>) / sizeof(VertRec));
+constexpr int VertCap = (int)((RowsBlock<RECS> / 64) * sizeof(ListLds) / sizeof(VertRec));
 
 // (a) + per-vertex (b) of vertex slot v of scene s (draw order: node,
 // instance, mesh vertex).
@@ -627,10 +630,9 @@ __device__ __forceinline__ void smooth_colour(const pbr_scene_desc& D, const Row
 // return rowstart[0..R] and u.ent[] hold them.
 template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsFrame& F, int s,
-                                            const RowsLds<SMOOTH, OM, RECS>& L, ListLds<RECS>& Q,
+                                            const RowsLds<SMOOTH, OM, RECS>& L, ListLds& Q,
                                             int lane, int jb0, int R, int r0, int r1, int rj0,
                                             int rj1) {
-  constexpr int kCap = Cap<RECS>;
   const bool mine = lane >= r0 && lane < r1;
   const int lo_row = max(rj0, jb0), hi_row = min(rj1, jb0 + R - 1);
   const int cnt = mine ? max(0, hi_row - lo_row + 1) : 0;
@@ -779,7 +781,6 @@ __device__ __forceinline__ void test_entry(const uint4& e, bool ok, int x0, cons
 template <int PXL, bool OM0, bool REF, int RECS>
 __device__ __forceinline__ void walk_row(const uint4* ent, uint32_t n0, int cnt, int emax, int x0,
                                          const float* pxf, PixState<PXL>& S) {
-  constexpr int kCap = Cap<RECS>;
   int k = 0;
   for (; k + 1 < emax; k += 2) {
     const bool ok0 = k < cnt, ok1 = k + 1 < cnt;
@@ -922,7 +923,7 @@ __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const Scen
 template <bool SMOOTH, int OM, int PXL, bool REF, int RECS>
 __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_frame_out& O,
                                              const RowsFrame& F, const Light& lt, int s,
-                                             const RowsLds<SMOOTH, OM, RECS>& L, ListLds<RECS>& Q,
+                                             const RowsLds<SMOOTH, OM, RECS>& L, ListLds& Q,
                                              int lane,
                                              int ja, int jb) {
   constexpr bool OM0 = OM == 0;
@@ -1019,22 +1020,22 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
 
 // D stays the first parameter: the out-of-line clipper reads it through the
 // kernarg segment (kernarg_scene).
-// Workgroups per CU an instantiation is built for (what its LDS and
-// registers allow): the 32-record flat form eight, or four with float32
-// colours; the 64-record form four, smooth forms with bf16 / float32 colours
-// three (their LDS) or two with 8-pixel lanes.
-template <bool SMOOTH, int OM, int PXL, int RECS>
-constexpr int rows_wgs_per_cu() {
-  if (RECS == 32) return OM == 2 ? 4 : 8;
+// Waves per SIMD an instantiation is built for (the second launch bound;
+// what its LDS allows): four (the 32-record form: eight two-wave workgroups
+// per CU; the 64-record form: four four-wave ones), smooth forms with bf16 /
+// float32 colours three, or two with 8-pixel lanes.
+template <bool SMOOTH, int OM, int PXL>
+constexpr int rows_waves_per_simd() {
   if (SMOOTH && OM != 0) return PXL == 8 ? 2 : 3;
   return 4;
 }
 
 template <bool SMOOTH, int OM, int PXL, bool REF, int RECS>
-__global__ __launch_bounds__(kRowsBlock, (rows_wgs_per_cu<SMOOTH, OM, PXL, RECS>())) void rows_kernel(
-    pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg) {
+__global__ __launch_bounds__(RowsBlock<RECS>, (rows_waves_per_simd<SMOOTH, OM, PXL>())) void
+rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg) {
+  constexpr int kBlock = RowsBlock<RECS>, kWaves = WavesPerScene<RECS>;
   __shared__ RowsLds<SMOOTH, OM, RECS> lds[2];
-  __shared__ ListLds<RECS> lists[kRowsBlock / 64];
+  __shared__ ListLds lists[kBlock / 64];
   const int tid = (int)threadIdx.x;
   // the wave index is uniform within the wave: keep it (and what derives
   // from it - scene, LDS region, output bases) in scalar registers
@@ -1052,7 +1053,7 @@ __global__ __launch_bounds__(kRowsBlock, (rows_wgs_per_cu<SMOOTH, OM, PXL, RECS>
     VertRec* vt = reinterpret_cast<VertRec*>(&lists[0]);
     const uint32_t nv0 = s0 < D.num_scenes ? scene_vertex_slots(D, s0) : 0u;
     const uint32_t nv1 = s0 + 1 < D.num_scenes ? scene_vertex_slots(D, s0 + 1) : 0u;
-    for (uint32_t v = (uint32_t)tid; v < nv0 + nv1; v += (uint32_t)kRowsBlock) {
+    for (uint32_t v = (uint32_t)tid; v < nv0 + nv1; v += (uint32_t)kBlock) {
       const int sv = v < nv0 ? 0 : 1;
       vertex_slot(D, F, s0 + sv, sv ? v - nv0 : v, vt[v]);
     }
@@ -1079,12 +1080,12 @@ __global__ __launch_bounds__(kRowsBlock, (rows_wgs_per_cu<SMOOTH, OM, PXL, RECS>
     }
   }
   __syncthreads();
-  // kWavesPerScene consecutive waves share scene s0 + w, each taking an
-  // equal share of its rows
-  const int sl = wave / kWavesPerScene, part = wave % kWavesPerScene;
+  // kWaves consecutive waves share scene s0 + w, each taking an equal share
+  // of its rows
+  const int sl = wave / kWaves, part = wave % kWaves;
   const int s = s0 + sl;
   if (s >= D.num_scenes) return;
-  const int ja = (F.H * part) / kWavesPerScene, jb = (F.H * (part + 1)) / kWavesPerScene;
+  const int ja = (F.H * part) / kWaves, jb = (F.H * (part + 1)) / kWaves;
   raster_scene<SMOOTH, OM, PXL, REF, RECS>(D, O, F, lt, s, lds[sl], lists[wave], lane, ja, jb);
 }
 
@@ -1094,10 +1095,10 @@ hipError_t launch_rows_t(const pbr_scene_desc* D, const pbr_frame_out* O, const 
   const int grid = (D->num_scenes + 1) / 2;
   // the 32-record form runs tiles of at most 64 columns (4-pixel lanes)
   if (RECS == 32 || D->width <= 64)
-    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 4, REF, RECS>), dim3(grid), dim3(kRowsBlock), 0,
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 4, REF, RECS>), dim3(grid), dim3(RowsBlock<RECS>), 0,
                        st, *D, *O, F, lt, bg);
   else if constexpr (RECS == 64)
-    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 8, REF, RECS>), dim3(grid), dim3(kRowsBlock), 0,
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 8, REF, RECS>), dim3(grid), dim3(RowsBlock<RECS>), 0,
                        st, *D, *O, F, lt, bg);
   return hipGetLastError();
 }
@@ -1132,13 +1133,12 @@ int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rende
   F.seg = 1;
   while (F.seg < slots) F.seg <<= 1;
   const int cap = opts ? opts->row_list_cap : 0;
-  if (cap != 0 && (cap < 8 || cap > Cap<64>))
-    return fail(PBR_ERR_INVALID, "row_list_cap %d outside [8, %d]", cap, Cap<64>);
+  if (cap != 0 && (cap < 8 || cap > kCap))
+    return fail(PBR_ERR_INVALID, "row_list_cap %d outside [8, %d]", cap, kCap);
   const bool smooth = any_smooth(D);
-  // flat scenes of at most 32 slots and 64 columns whose list cap fits run
-  // the 32-record form
-  const bool small = !smooth && slots <= 32 && D->width <= 64 && cap <= Cap<32>;
-  F.cap = cap ? cap : (small ? Cap<32> : Cap<64>);
+  // flat scenes of at most 32 slots and 64 columns run the 32-record form
+  const bool small = !smooth && slots <= 32 && D->width <= 64;
+  F.cap = cap ? cap : kCap;
   // the vertex phase needs both scenes' vertex slots in the table
   uint64_t vslots = 0;
   for (int j = 0; j < D->num_nodes; ++j)
