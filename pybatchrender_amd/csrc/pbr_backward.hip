@@ -61,8 +61,11 @@ constexpr int kAcc = 32;   // accumulator floats per slot
 constexpr int kRecF = 24;  // per-pixel factors per slot
 constexpr int kLightF = 10;
 constexpr int kMaxCap = 512;          // flat kernel: slots held in LDS
-// per-scene kernel: slots held in LDS per pass (two workgroups fit a CU)
-constexpr int kPassCap = 320;
+// per-scene kernel: slots held in LDS per pass (43 KiB of factors and
+// accumulators: three workgroups fit a CU).  Measured on MI355X for the
+// 512-scene 128 x 128 smooth workload's backward: 128 -> 525 us, 160 -> 462,
+// 192 -> 442, 224 -> 541, 320 -> 502 (two workgroups per CU).
+constexpr int kPassCap = 192;
 // per-scene kernel: rows per band when a scene is split over several
 // workgroups, and the workgroup count up to which scenes are split
 constexpr int kBandRows = 16;
