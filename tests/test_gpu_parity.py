@@ -410,6 +410,36 @@ def test_rows_kernel_small_scenes(cuda, res, cap, dtype):
                  vis=False)
 
 
+@pytest.mark.parametrize("dtype", ["uint8", "float32"])
+def test_rows_kernel_small_scenes_instance_counts(cuda, dtype):
+    """The 32-record form with per-scene instance counts (0-2 boxes of one
+    node and 0-1 of a shared quad: at most 26 slots, some scenes empty), then
+    counts edited in place."""
+    n = 40
+    gen = torch.Generator("cpu").manual_seed(38)
+    r = PBRRenderer(PBRConfig(num_scenes=n, tile_resolution=(64, 48), device="cuda",
+                              kernel="rows", output_dtype=dtype))
+    a = r.add_node("models/box", instances_per_scene=2)
+    a.set_transforms(workloads.random_transforms(n * 2, gen, pos_range=2.0))
+    a.set_colors(torch.rand(n * 2, 4, generator=gen))
+    a.set_instance_counts(torch.randint(0, 3, (n,), generator=gen))
+    b = r.add_node("models/quad", instances_per_scene=1, shared_across_scenes=True)
+    b.set_transforms(workloads.random_transforms(1, gen, pos_range=1.0))
+    b.set_colors(torch.rand(1, 4, generator=gen))
+    b.set_instance_counts(torch.randint(0, 2, (n,), generator=gen))
+    r.add_camera()
+    eye = torch.stack([torch.rand(n, generator=gen) * 4 - 2, torch.full((n,), -6.0),
+                       torch.rand(n, generator=gen) * 3], 1)
+    r._pbr_cam.set_positions_and_lookat(eye, torch.zeros(3))
+    r.add_light()
+    r.setup_environment()
+    check_parity(r, f"rows kernel 26-slot node counts {dtype}")
+    check_parity(r, f"rows kernel counts {dtype} colour only", depth=False, vis=False)
+    a.instance_counts[::3] = 2
+    b.instance_counts[1::2] = 0
+    check_parity(r, f"rows kernel counts edited in place {dtype}")
+
+
 @pytest.mark.parametrize("cap", [0, 8])
 def test_rows_kernel_small_scene_clipping(cuda, cap):
     """The 32-record form with triangles through the near plane (two large
