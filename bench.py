@@ -2,9 +2,10 @@
 
 Workload: the 64x64 CartPole scene with cart, pole and a shared ground quad
 (CartPoleGroundRenderer), 4096 scenes, uint8 RGB output [4096, 3, 64, 64].
-A step is one renderer.step(state): state [4096, 4] -> cart/pole instance
-transforms on the device (one pose launch) -> one row-span render launch
-(setup, row lists and pixel walk of every scene).
+A step is one renderer.step(state): one row-span launch (pbr_render_posed)
+that evaluates the cart/pole instance transforms of each scene from the
+state [4096, 4], writes them to the nodes' buffers and renders (setup, row
+lists and pixel walk of every scene).
 States are seeded (x ~ U(-2.4, 2.4), theta ~ U(-0.21, 0.21)), one batch per
 step, and resident on the GPU before timing starts.
 
@@ -13,10 +14,13 @@ Printed JSON line (one, from this single process):
                (synchronize on both sides); `graphed_step` is the same step
                captured once as a hipGraph and replayed (it adds the copy of
                each state batch into the graph's static input)
-  roofline     the render call (one row-span kernel launch): algorithmic
-               bytes per call (uint8 RGB pixels + 80 B per dynamic instance +
-               64 B VP per scene) / average time of the call from HIP events
-               around back-to-back calls on the launch stream, vs 8 TB/s
+  roofline     the step's launch (one row-span kernel that also poses the
+               nodes): algorithmic bytes per call (uint8 RGB pixels + 80 B
+               per dynamic instance + 64 B VP per scene, + per posed row the
+               rot3 / scale outputs and the pose components read) / average
+               time of the call from HIP events around back-to-back calls on
+               the launch stream, vs 8 TB/s; `render_only` is the render
+               call of the same posed scene without the poses
   cpu_baseline the CPU oracle (a C restatement of the same render path)
                rendering a 256-scene sample of the same workload on all host
                cores: median render (p10/p90 beside it) and a one-thread
@@ -113,9 +117,37 @@ def config_renderer(name: str, persistent: bool):
     raise ValueError(name)
 
 
-def kernel_time_us(r, launches: int = 200) -> float:
+def step_poses(r, state) -> list:
+    """The pose descriptors renderer.step(state) hands to its render call
+    (renderers with fuse_poses), with the tensors they point to; [] for
+    renderers that render without poses."""
+    if state is None or not getattr(r, "fuse_poses", False):
+        return []
+    r._pending_poses = []
+    try:
+        r._step(state)
+        return list(r._pending_poses)
+    finally:
+        r._pending_poses = None
+
+
+def pose_bytes(poses: list) -> int:
+    """Algorithmic bytes a fused step adds to the render's: per posed row the
+    rot3 (36 B) and scale (4 B) outputs and 4 B per component read from a
+    source (the 64-byte transforms row, which the render counts as read, is
+    written instead)."""
+    total = 0
+    for _, (d, _keep) in poses:
+        n_src = sum(1 for k in "xyzhprs" if getattr(d, k).src)
+        total += d.rows * ((36 if d.rot3 else 0) + (4 if d.scale else 0) + 4 * n_src)
+    return total
+
+
+def kernel_time_us(r, launches: int = 200, state=None) -> float:
     """Average render-kernel duration from HIP events bracketing back-to-back
-    launches of one prebuilt descriptor on the current stream."""
+    launches of one prebuilt descriptor on the current stream: the launch
+    renderer.step(state) makes (pbr_render_posed with the step's poses for
+    fuse_poses renderers), or the plain render without a state."""
     import torch
 
     from pybatchrender import _abi, _native
@@ -136,21 +168,29 @@ def kernel_time_us(r, launches: int = 200) -> float:
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
-    args = (ctypes.byref(desc), ctypes.byref(fo), ctypes.byref(opts), ctypes.c_void_p(ws.data_ptr()),
-            ws_bytes, sp)
+    poses = step_poses(r, state)
+    if poses:
+        arr = (_abi.pbr_pose_desc * len(poses))(*[d for _, (d, _k) in poses])
+        fn = lib.pbr_render_posed
+        args = (ctypes.byref(desc), ctypes.byref(fo), ctypes.byref(opts), arr, len(poses),
+                ctypes.c_void_p(ws.data_ptr()), ws_bytes, sp)
+    else:
+        fn = lib.pbr_render
+        args = (ctypes.byref(desc), ctypes.byref(fo), ctypes.byref(opts),
+                ctypes.c_void_p(ws.data_ptr()), ws_bytes, sp)
     for _ in range(20):
-        lib.pbr_render(*args)
+        fn(*args)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(launches):
-        rc = lib.pbr_render(*args)
+        rc = fn(*args)
         if rc:
             raise RuntimeError(_native.last_error())
     e1.record(stream)
     e1.synchronize()
-    del keep
+    del keep, poses
     return e0.elapsed_time(e1) * 1000.0 / launches
 
 
@@ -596,8 +636,10 @@ def main() -> int:
     if args.kernel_only:
         log("[bench] kernel timing only")
         r.step(states[0])
-        kus = kernel_time_us(r, launches=args.steps)
-        print(json.dumps({"kernel_us": kus, "bytes_per_launch": algorithmic_bytes(r)}), flush=True)
+        kus = kernel_time_us(r, launches=args.steps, state=states[0])
+        nb = algorithmic_bytes(r) + pose_bytes(step_poses(r, states[0]))
+        print(json.dumps({"kernel_us": kus, "bytes_per_launch": nb,
+                          "render_only_us": kernel_time_us(r, launches=args.steps)}), flush=True)
         if args.copy_calibration:
             copy_calibration()
         return 0
@@ -634,9 +676,12 @@ def main() -> int:
     graphed = n * args.steps / (time.perf_counter() - t0)
 
     log("[bench] kernel timing")
-    kus = kernel_time_us(r)
-    nbytes = algorithmic_bytes(r)
+    # the step's one launch: pose + render (pbr_render_posed)
+    kus = kernel_time_us(r, state=states[0])
+    nbytes = algorithmic_bytes(r) + pose_bytes(step_poses(r, states[0]))
     achieved = nbytes / (kus * 1e-6) / 1e9
+    render_us = kernel_time_us(r)
+    render_bytes = algorithmic_bytes(r)
 
     rec = {
         "metric": "scene-frames/s, 4096-env CartPole 64x64, 1 GPU; fwd+bwd vs plain fwd; HBM %",
@@ -657,8 +702,12 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(render_kernel_name(r)),
-                     "kernel": KERNEL_LABEL[render_kernel_name(r)], "kernel_us": round(kus, 3),
-                     "bytes_per_launch": nbytes},
+                     "kernel": KERNEL_LABEL[render_kernel_name(r)] + ", posing the step's nodes",
+                     "kernel_us": round(kus, 3), "bytes_per_launch": nbytes,
+                     "render_only": {"kernel_us": round(render_us, 3),
+                                     "bytes_per_launch": render_bytes,
+                                     "frac": round(render_bytes / render_us / 1e3 / HBM_PEAK_GBS,
+                                                   4)}},
     }
     rec["graphed_step"] = {"scene_frames_per_s": round(graphed, 1),
                            "note": "PBRRenderer.capture_step replay incl. state copy"}

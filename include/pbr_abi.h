@@ -237,6 +237,21 @@ int pbr_set_pose(const pbr_pose_desc* pose, void* stream);
 /* pbr_set_pose for `count` (<= 8) nodes in one launch. */
 int pbr_set_poses(const pbr_pose_desc* poses, int32_t count, void* stream);
 
+/* pbr_set_poses(poses, npose) followed by pbr_render, on `stream`: one
+ * renderer step whose _step only poses nodes.  Replaces renderer.py:377-389
+ * (step: _step, here the node.py:116-154 setters, then the render).  When
+ * the row-span kernel draws the batch and every pose with rows writes the
+ * transforms of its own unshared node of `scene` (same pointer, rows =
+ * num_scenes * instances_per_scene, at most 64 posed rows per scene, no
+ * pose output overlapping what the render reads), one launch applies the
+ * poses and draws; otherwise the two launches run.  Outputs (pixels, the
+ * pose rows, rot3, scale) are the same either way.  Errors are those of
+ * the two calls; after a render error the poses may have been applied. */
+int pbr_render_posed(const pbr_scene_desc* scene, const pbr_frame_out* out,
+                     const pbr_render_opts* opts, const pbr_pose_desc* poses,
+                     int32_t npose, void* workspace, size_t workspace_bytes,
+                     void* stream);
+
 /* One CartPole env step of every scene in one launch: the Euler step of the
  * cart-pole equations, termination, reward and auto-reset
  * (envs/cartpole/env.py:111-139 _dynamics/_termination/_reward and

@@ -211,6 +211,7 @@ EXPORTS = (
     "pbr_backward_workspace_bytes",
     "pbr_set_pose",
     "pbr_set_poses",
+    "pbr_render_posed",
     "pbr_cartpole_step",
     "pbr_cartpole_reset",
     "pbr_workspace_bytes",
@@ -261,6 +262,17 @@ def declare(lib: C.CDLL) -> C.CDLL:
     lib.pbr_set_pose.restype = C.c_int
     lib.pbr_set_poses.argtypes = [C.POINTER(pbr_pose_desc), C.c_int32, C.c_void_p]
     lib.pbr_set_poses.restype = C.c_int
+    lib.pbr_render_posed.argtypes = [
+        C.POINTER(pbr_scene_desc),
+        C.POINTER(pbr_frame_out),
+        C.POINTER(pbr_render_opts),
+        C.POINTER(pbr_pose_desc),
+        C.c_int32,
+        C.c_void_p,
+        C.c_size_t,
+        C.c_void_p,
+    ]
+    lib.pbr_render_posed.restype = C.c_int
     lib.pbr_cartpole_step.argtypes = [C.POINTER(pbr_cartpole_params), C.POINTER(pbr_cartpole_batch),
                                       C.POINTER(pbr_pose_desc), C.c_int32, C.c_void_p]
     lib.pbr_cartpole_step.restype = C.c_int

@@ -55,6 +55,20 @@ def render(desc: _abi.pbr_scene_desc, out: _abi.pbr_frame_out, opts: _abi.pbr_re
         raise RuntimeError(f"pbr_render failed (status {rc}): {last_error()}")
 
 
+def render_posed(desc: _abi.pbr_scene_desc, out: _abi.pbr_frame_out, opts: _abi.pbr_render_opts,
+                 pose_descs: list, workspace_ptr: int, workspace_bytes: int,
+                 stream_ptr: int) -> None:
+    """pbr_render_posed: pbr_set_poses(pose_descs) then pbr_render, one launch
+    when the row-span kernel can apply the poses itself."""
+    arr = (_abi.pbr_pose_desc * len(pose_descs))(*pose_descs)
+    rc = library().pbr_render_posed(
+        ctypes.byref(desc), ctypes.byref(out), ctypes.byref(opts), arr, len(pose_descs),
+        ctypes.c_void_p(workspace_ptr or None), workspace_bytes, ctypes.c_void_p(stream_ptr or None),
+    )
+    if rc != _abi.PBR_OK:
+        raise RuntimeError(f"pbr_render_posed failed (status {rc}): {last_error()}")
+
+
 def workspace_bytes(desc: _abi.pbr_scene_desc, opts: _abi.pbr_render_opts) -> int:
     n = ctypes.c_size_t(0)
     rc = library().pbr_workspace_bytes(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(n))

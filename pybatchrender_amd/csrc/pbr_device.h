@@ -604,33 +604,45 @@ __device__ __forceinline__ void write_dead(uint32_t id, GRec* g) {
 // ---------------------------------------------------------------------------
 constexpr int kMaxPoseBatch = 8;
 
+// Component c at row i.  The source is read with a global (not flat) load:
+// a flat load also counts against the counter that every scalar read of the
+// descriptor waits on, which would make each component's load wait for the
+// previous one.
 __device__ __forceinline__ float pose_value(const pbr_pose_component& c, int64_t i) {
-  return c.src ? c.src[i * c.stride] : c.value;
+  typedef const __attribute__((address_space(1))) float gfloat;
+  return c.src ? ((gfloat*)c.src)[i * c.stride] : c.value;
 }
 
-// cos and sin of angle component c at row i; a constant +0 angle skips the
-// evaluation (cosf(+0) = 1 and sinf(+0) = +0 exactly).
-__device__ __forceinline__ void pose_cos_sin(const pbr_pose_component& c, int64_t i, float& cs,
+// The seven components (x, y, z, h, p, r, s) of row i, loaded before any
+// arithmetic so the loads overlap.
+__device__ __forceinline__ void pose_inputs(const pbr_pose_desc& P, int64_t i, float v[7]) {
+  v[0] = pose_value(P.x, i);
+  v[1] = pose_value(P.y, i);
+  v[2] = pose_value(P.z, i);
+  v[3] = pose_value(P.h, i);
+  v[4] = pose_value(P.p, i);
+  v[5] = pose_value(P.r, i);
+  v[6] = pose_value(P.s, i);
+}
+
+// cos and sin of angle component c whose row value is a; a constant +0
+// angle skips the evaluation (cosf(+0) = 1 and sinf(+0) = +0 exactly).
+__device__ __forceinline__ void pose_cos_sin(const pbr_pose_component& c, float a, float& cs,
                                              float& sn) {
   if (!c.src && __float_as_uint(c.value) == 0u) {
     cs = 1.0f;
     sn = 0.0f;
     return;
   }
-  const float a = pose_value(c, i);
   cs = cosf(a);
   sn = sinf(a);
 }
 
-// Row i of P: rows 0-2 of M, R and s.
-__device__ __forceinline__ void pose_row(const pbr_pose_desc& P, int64_t i, float4 M[3],
-                                         float R[9], float& sc) {
-  const float x = pose_value(P.x, i), y = pose_value(P.y, i), z = pose_value(P.z, i);
-  sc = pose_value(P.s, i);
-  float ch, sh, cp, sp, cr, sr;
-  pose_cos_sin(P.h, i, ch, sh);
-  pose_cos_sin(P.p, i, cp, sp);
-  pose_cos_sin(P.r, i, cr, sr);
+// Rows 0-2 of M, R and s from the components v and the angles' cos / sin.
+__device__ __forceinline__ void pose_matrix(const float v[7], float ch, float sh, float cp,
+                                            float sp, float cr, float sr, float4 M[3],
+                                            float R[9], float& sc) {
+  sc = v[6];
   const float chsp = ch * sp, shsp = sh * sp;
   R[0] = ch * cp;
   R[1] = chsp * sr - sh * cr;
@@ -641,9 +653,21 @@ __device__ __forceinline__ void pose_row(const pbr_pose_desc& P, int64_t i, floa
   R[6] = -sp;
   R[7] = cp * sr;
   R[8] = cp * cr;
-  M[0] = make_float4(R[0] * sc, R[1] * sc, R[2] * sc, x);
-  M[1] = make_float4(R[3] * sc, R[4] * sc, R[5] * sc, y);
-  M[2] = make_float4(R[6] * sc, R[7] * sc, R[8] * sc, z);
+  M[0] = make_float4(R[0] * sc, R[1] * sc, R[2] * sc, v[0]);
+  M[1] = make_float4(R[3] * sc, R[4] * sc, R[5] * sc, v[1]);
+  M[2] = make_float4(R[6] * sc, R[7] * sc, R[8] * sc, v[2]);
+}
+
+// Row i of P: rows 0-2 of M, R and s.
+__device__ __forceinline__ void pose_row(const pbr_pose_desc& P, int64_t i, float4 M[3],
+                                         float R[9], float& sc) {
+  float v[7];
+  pose_inputs(P, i, v);
+  float ch, sh, cp, sp, cr, sr;
+  pose_cos_sin(P.h, v[3], ch, sh);
+  pose_cos_sin(P.p, v[4], cp, sp);
+  pose_cos_sin(P.r, v[5], cr, sr);
+  pose_matrix(v, ch, sh, cp, sp, cr, sr, M, R, sc);
 }
 
 // Stores row i of P's outputs (transforms, and rot3 / scale when present).

@@ -41,9 +41,15 @@ int launch_tiles(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rend
                  void* workspace, size_t workspace_bytes, hipStream_t st);
 
 // The row-span kernel (pbr_rows.hip): whether it serves the descriptor, and
-// its launch (no workspace).
+// its launch (no workspace).  With `npose` > 0 the launch first applies
+// `poses` as pbr_set_poses would (each workgroup the rows of its own scenes)
+// and draws the posed scene; rows_can_pose says whether it can (each pose
+// writes the transforms of a distinct unshared node of D, rows = num_scenes
+// * ips, at most 64 posed rows per scene, no pose output overlapping what
+// the launch reads).
 bool rows_supported(const pbr_scene_desc* D);
+bool rows_can_pose(const pbr_scene_desc* D, const pbr_pose_desc* poses, int32_t count);
 int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
-                hipStream_t st);
+                hipStream_t st, const pbr_pose_desc* poses = nullptr, int32_t npose = 0);
 
 }  // namespace pbr_host

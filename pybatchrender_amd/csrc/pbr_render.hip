@@ -1257,6 +1257,25 @@ int pbr_render(const pbr_scene_desc* scene, const pbr_frame_out* out, const pbr_
   }
 }
 
+int pbr_render_posed(const pbr_scene_desc* scene, const pbr_frame_out* out,
+                     const pbr_render_opts* opts, const pbr_pose_desc* poses, int32_t npose,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = validate(scene, out);
+  if (rc) return rc;
+  rc = check_poses(poses, npose, nullptr);
+  if (rc) return rc;
+  const int kernel = opts ? opts->kernel : PBR_KERNEL_AUTO;
+  if (kernel < PBR_KERNEL_AUTO || kernel > PBR_KERNEL_ROWS)
+    return fail(PBR_ERR_INVALID, "unknown kernel %d", kernel);
+  g_err[0] = 0;
+  if (npose > 0 && scene->num_scenes > 0 && choose_kernel(scene, opts) == PBR_KERNEL_ROWS &&
+      rows_supported(scene) && rows_can_pose(scene, poses, npose))
+    return launch_rows(scene, out, opts, (hipStream_t)stream, poses, npose);
+  rc = pbr_set_poses(poses, npose, stream);
+  if (rc) return rc;
+  return pbr_render(scene, out, opts, workspace, workspace_bytes, stream);
+}
+
 int pbr_render_noalloc(const pbr_scene_desc* scene, const pbr_frame_out* out,
                        const pbr_render_opts* opts, void* stream) {
   int rc = validate(scene, out);

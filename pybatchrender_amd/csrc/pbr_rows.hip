@@ -34,6 +34,9 @@
 // Coverage, depth and colour are bit-identical to the CPU oracle.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+
+#include <type_traits>
 
 #include "../../include/pbr_abi.h"
 #include "pbr_device.h"
@@ -69,6 +72,38 @@ struct RowsFrame {
   int cap;  // row-list entries per band (8..kCap)
   int vphase;  // 1: vertices transformed once into an LDS table (they fit)
 };
+
+// Instance poses a launch applies itself (pbr_render_posed).  Pose d[k]
+// belongs to an unshared node of ips[k] instances per scene; within a scene
+// the posed rows are numbered 0 .. rows - 1, pose k's from first[k], and
+// node_row[j] is the first posed row of node j (-1: not posed).  Each
+// workgroup evaluates the rows of its two scenes (rows s * ips[k] ..
+// s * ips[k] + ips[k] - 1 of pose k) into LDS before its vertex phase, with
+// the arithmetic of pose_kernel, and stores them to the node's transforms
+// (and rot3 / scale) after its last barrier.
+constexpr int kFusedRows = 16;  // posed rows per scene at most
+constexpr int kFusedPoses = 4;  // poses at most: one per wave of the 64-record form
+struct RowsPoses {
+  int32_t count, rows;
+  int32_t ips[kFusedPoses], first[kFusedPoses];
+  int32_t node_row[PBR_MAX_NODES];
+  pbr_pose_desc d[kFusedPoses];
+};
+// A workgroup's posed rows in LDS, per scene: rows 0-2 of M, then R and s
+// (the rot3 / scale outputs); the scenes' view-projections; node_row as in
+// RowsPoses.
+struct PoseRows {
+  float4 m[kFusedRows][3];
+  float rs[kFusedRows][10];
+};
+struct PoseLds {
+  PoseRows sc[2];
+  float4 vp[2][4];
+  int8_t node_row[PBR_MAX_NODES];
+};
+struct NoPoses {};
+template <bool POSED>
+using PoseArg = typename std::conditional<POSED, RowsPoses, NoPoses>::type;
 
 // Unclipped: edge coefficients, depth plane and rows of a snapped triangle.
 // Clipped: A[0] = node, A[1] = slot within the node, rows = union of the
@@ -278,12 +313,15 @@ __device__ __forceinline__ int locate_slot(const pbr_scene_desc& D, int s, uint3
 // (a) of slot (node, local) of scene s: the node's fields are picked from a
 // uniform walk over the nodes, so the dependent loads are the instance row,
 // the scene's view-projection, the triangle's indices and then its vertices.
+// With PL (a launch that poses nodes itself, after its vertex phase) a posed
+// node's matrix comes from the workgroup's LDS rows, the bottom row being
+// (0, 0, 0, 1) as pose_kernel writes it.
 __device__ __forceinline__ void fetch_slot(const pbr_scene_desc& D, int s, int node,
                                            uint32_t local, CVert cv[3], float rgba[4],
-                                           bool& flat) {
+                                           bool& flat, const PoseLds* PL = nullptr) {
   const float* trans = nullptr;
   const float* cols = nullptr;
-  int ips = 1, shared = 0, voff = 0, ioff = 0, ntri = 1, fl = 0;
+  int ips = 1, shared = 0, voff = 0, ioff = 0, ntri = 1, fl = 0, prow = -1;
   for (int j = 0; j < D.num_nodes; ++j) {
     if (node == j) {
       const pbr_node& nd = D.nodes[j];
@@ -296,12 +334,25 @@ __device__ __forceinline__ void fetch_slot(const pbr_scene_desc& D, int s, int n
       voff = me.vertex_offset;
       ioff = me.index_offset;
       ntri = me.num_triangles;
+      if (PL) prow = PL->node_row[j];
     }
   }
   const uint32_t i = local / (uint32_t)ntri, t = local - i * (uint32_t)ntri;
   const size_t row = shared ? (size_t)i : (size_t)s * (size_t)ips + (size_t)i;
-  const float4* M4 = reinterpret_cast<const float4*>(trans + row * 16);
-  const float4 m0 = M4[0], m1 = M4[1], m2 = M4[2], m3 = M4[3];
+  float4 m0, m1, m2, m3;
+  if (PL && prow >= 0) {
+    const float4* P = PL->sc[s & 1].m[prow + (int)i];
+    m0 = P[0];
+    m1 = P[1];
+    m2 = P[2];
+    m3 = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  } else {
+    const float4* M4 = reinterpret_cast<const float4*>(trans + row * 16);
+    m0 = M4[0];
+    m1 = M4[1];
+    m2 = M4[2];
+    m3 = M4[3];
+  }
   const float M[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
                        m2.x, m2.y, m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
   const float4 c4 = reinterpret_cast<const float4*>(cols)[row];
@@ -363,11 +414,12 @@ __device__ __forceinline__ void edges_of(const FixTri& T, RowRec& R) {
 // clipper then keep their registers out of the row-span kernel's own.
 __device__ __noinline__ int clipped_row_entries(const pbr_scene_desc* D, RowsFrame F, int s,
                                                 int node, uint32_t local, int j, uint4* ent,
-                                                int room, uint32_t rref, uint32_t colw) {
+                                                int room, uint32_t rref, uint32_t colw,
+                                                const PoseLds* PL) {
   CVert cv[3];
   float rgba[4];
   bool flat;
-  fetch_slot(*D, s, node, local, cv, rgba, flat);
+  fetch_slot(*D, s, node, local, cv, rgba, flat, PL);
   if (ent == nullptr)
     return clipped_spans(cv, F, j, false, nullptr, nullptr, nullptr, nullptr, nullptr);
   int lo[kMaxPoly], hi[kMaxPoly], qs[kMaxPoly];
@@ -450,11 +502,13 @@ template <int RECS>
 constexpr int VertCap = (int)((RowsBlock<RECS> / 64) * sizeof(ListLds) / sizeof(VertRec));
 
 // (a) + per-vertex (b) of vertex slot v of scene s (draw order: node,
-// instance, mesh vertex).
+// instance, mesh vertex).  With PL (a launch that poses nodes itself) the
+// matrices of posed nodes and the view-projection come from the
+// workgroup's LDS.
 __device__ __forceinline__ void vertex_slot(const pbr_scene_desc& D, const RowsFrame& F, int s,
-                                            uint32_t v, VertRec& out) {
+                                            uint32_t v, VertRec& out, const PoseLds* PL) {
   const float* trans = nullptr;
-  int ips = 1, shared = 0, voff = 0, nv = 1;
+  int ips = 1, shared = 0, voff = 0, nv = 1, prow = -1;
   uint32_t local = 0, acc = 0;
   bool found = false;
   for (int j = 0; j < D.num_nodes; ++j) {
@@ -471,6 +525,7 @@ __device__ __forceinline__ void vertex_slot(const pbr_scene_desc& D, const RowsF
       shared = nd.shared;
       voff = me.vertex_offset;
       nv = me.num_vertices;
+      if (PL) prow = PL->node_row[j];
     }
     acc += n;
   }
@@ -479,8 +534,26 @@ __device__ __forceinline__ void vertex_slot(const pbr_scene_desc& D, const RowsF
   const uint32_t i = local / (uint32_t)nv, k = local - i * (uint32_t)nv;
   const size_t row = shared ? (size_t)i : (size_t)s * (size_t)ips + (size_t)i;
   float M[16], VP[16];
-  load_mat4(trans + row * 16, M);
-  load_mat4(D.view_proj + (size_t)s * 16, VP);
+  if (PL && prow >= 0) {
+    const float4* P = PL->sc[s & 1].m[prow + (int)i];
+    const float4 m0 = P[0], m1 = P[1], m2 = P[2];
+    const float Mp[16] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w,
+                          m2.x, m2.y, m2.z, m2.w, 0.0f, 0.0f, 0.0f, 1.0f};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) M[q] = Mp[q];
+  } else {
+    load_mat4(trans + row * 16, M);
+  }
+  if (PL) {
+    const float4* V = PL->vp[s & 1];
+    const float4 q0 = V[0], q1 = V[1], q2 = V[2], q3 = V[3];
+    const float Vp[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) VP[q] = Vp[q];
+  } else {
+    load_mat4(D.view_proj + (size_t)s * 16, VP);
+  }
   vertex_record(M, VP, D.vertices + (size_t)(voff + (int)k) * 6, F.W, F.H, F.guard, out);
 }
 
@@ -490,7 +563,8 @@ __device__ __forceinline__ void vertex_slot(const pbr_scene_desc& D, const RowsF
 template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void setup_slot_v(const pbr_scene_desc& D, const RowsFrame& F,
                                              const Light& lt, int s, uint32_t slot,
-                                             RowsLds<SMOOTH, OM, RECS>& L, const VertRec* vt) {
+                                             RowsLds<SMOOTH, OM, RECS>& L, const VertRec* vt,
+                                             const PoseLds* PL) {
   uint32_t local, total;
   const int node = locate_slot(D, s, slot, local, total);
   if (slot == 0) L.nrec = (int)total;
@@ -551,7 +625,7 @@ __device__ __forceinline__ void setup_slot_v(const pbr_scene_desc& D, const Rows
       CVert cv[3];
       float rgba2[4];
       bool flat2;
-      fetch_slot(D, s, node, local, cv, rgba2, flat2);
+      fetch_slot(D, s, node, local, cv, rgba2, flat2, PL);
       int j0, j1;
       const int n = clipped_rows(cv, F, j0, j1);
       if (j0 <= j1) {
@@ -603,7 +677,8 @@ template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void smooth_colour(const pbr_scene_desc& D, const RowsFrame& F,
                                               const Light& lt, int s,
                                               const RowsLds<SMOOTH, OM, RECS>& L, uint32_t ref, int x,
-                                              int j, float col[4]) {
+                                              int j, float col[4],
+                                              const PoseLds* PL) {
   const uint32_t slot = ref >> 3;
   const uint32_t fl = L.flags[slot];
   const RowRec& R = L.rec[slot];
@@ -612,7 +687,7 @@ __device__ __forceinline__ void smooth_colour(const pbr_scene_desc& D, const Row
     CVert cv[3];
     float rgba[4];
     bool flat;
-    fetch_slot(D, s, R.A[0], (uint32_t)R.A[1], cv, rgba, flat);
+    fetch_slot(D, s, R.A[0], (uint32_t)R.A[1], cv, rgba, flat, PL);
     clipped_smooth_colour(cv, rgba, F, lt, (int)(ref & 7u), px, py, col);
     return;
   }
@@ -632,7 +707,7 @@ template <bool SMOOTH, int OM, int RECS>
 __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsFrame& F, int s,
                                             const RowsLds<SMOOTH, OM, RECS>& L, ListLds& Q,
                                             int lane, int jb0, int R, int r0, int r1, int rj0,
-                                            int rj1) {
+                                            int rj1, const PoseLds* PL) {
   const bool mine = lane >= r0 && lane < r1;
   const int lo_row = max(rj0, jb0), hi_row = min(rj1, jb0 + R - 1);
   const int cnt = mine ? max(0, hi_row - lo_row + 1) : 0;
@@ -676,7 +751,7 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
     int c = 0;
     if (fl & kRFlagClipped) {
       c = clipped_row_entries(kernarg_scene(), F, s, Rr.A[0], (uint32_t)Rr.A[1], j, nullptr, 0,
-                              0u, 0u);
+                              0u, 0u, PL);
     } else {
       pbr_span::Edges E;
 #pragma unroll
@@ -727,7 +802,7 @@ __device__ __forceinline__ void build_lists(const pbr_scene_desc& D, const RowsF
     if (L.flags[r] & kRFlagClipped) {
       if (pos < (uint32_t)kCap)
         clipped_row_entries(kernarg_scene(), F, s, Rr.A[0], (uint32_t)Rr.A[1], jb0 + b,
-                            &Q.u.ent[pos], kCap - (int)pos, (uint32_t)r << 3, colw);
+                            &Q.u.ent[pos], kCap - (int)pos, (uint32_t)r << 3, colw, PL);
     } else if (pos < (uint32_t)kCap) {
       const float tz = fmaf(Rr.zy, (float)(jb0 + b) + 0.5f, Rr.zc);
       Q.u.ent[pos] = make_uint4((uint32_t)plh[it] | ((uint32_t)r << 19), __float_as_uint(tz),
@@ -819,7 +894,7 @@ template <bool SMOOTH, int OM, int PXL, int RECS>
 __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const SceneOut& so,
                                              const RowsFrame& F, const Light& lt, int s,
                                              const RowsLds<SMOOTH, OM, RECS>& L, int x0, int j,
-                                             PixState<PXL>& S) {
+                                             PixState<PXL>& S, const PoseLds* PL) {
   if (x0 >= F.W) return;
   const uint32_t pix0 = (uint32_t)(F.H - 1 - j) * (uint32_t)F.W + (uint32_t)x0;
   typename ColOf<OM>::T cv[PXL];
@@ -833,7 +908,7 @@ __device__ __forceinline__ void resolve_lane(const pbr_scene_desc& D, const Scen
     }
     if (SMOOTH && slot != (uint32_t)RECS && (L.flags[slot] & kRFlagSmooth)) {
       float c4[4];
-      smooth_colour<SMOOTH, OM, RECS>(D, F, lt, s, L, S.ref[i], x0 + i, j, c4);
+      smooth_colour<SMOOTH, OM, RECS>(D, F, lt, s, L, S.ref[i], x0 + i, j, c4, PL);
       cv[i] = pack_col<OM>(c4);
     }
   }
@@ -924,8 +999,7 @@ template <bool SMOOTH, int OM, int PXL, bool REF, int RECS>
 __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_frame_out& O,
                                              const RowsFrame& F, const Light& lt, int s,
                                              const RowsLds<SMOOTH, OM, RECS>& L, ListLds& Q,
-                                             int lane,
-                                             int ja, int jb) {
+                                             int lane, int ja, int jb, const PoseLds* PL) {
   constexpr bool OM0 = OM == 0;
   const int nrec = __builtin_amdgcn_readfirstlane(L.nrec);
   int rj0 = 1, rj1 = 0, rw = 1;
@@ -957,7 +1031,7 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
       R = (R + 1) >> 1;
     }
     if (E <= F.cap) {
-      build_lists<SMOOTH, OM, RECS>(D, F, s, L, Q, lane, jb0, R, 0, nrec, rj0, rj1);
+      build_lists<SMOOTH, OM, RECS>(D, F, s, L, Q, lane, jb0, R, 0, nrec, rj0, rj1, PL);
       // per-lane constants of the row walk, made after the lists so they are
       // not held through them
       const int x0 = (lane & 15) * PXL;
@@ -981,7 +1055,7 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
         PixState<PXL> S;
         init_state<PXL, RECS>(S, bg);
         walk_row<PXL, OM0, REF, RECS>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
-        if (row_ok) resolve_lane<SMOOTH, OM, PXL, RECS>(D, so, F, lt, s, L, x0, jb0 + b, S);
+        if (row_ok) resolve_lane<SMOOTH, OM, PXL, RECS>(D, so, F, lt, s, L, x0, jb0 + b, S, PL);
       }
       wave_sync();
     } else {
@@ -1005,14 +1079,14 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
         if (!m) continue;
         const int r0 = __ffsll((unsigned long long)m) - 1;
         const int r1 = 64 - __clzll((long long)m);
-        build_lists<SMOOTH, OM, RECS>(D, F, s, L, Q, lane, j, 1, r0, r1, rj0, rj1);
+        build_lists<SMOOTH, OM, RECS>(D, F, s, L, Q, lane, j, 1, r0, r1, rj0, rj1, PL);
         const uint32_t n0 = Q.rowstart[0];
         const int cnt = q == 0 ? (int)(Q.rowstart[1] - n0) : 0;
         const int emax = __builtin_amdgcn_readlane(cnt, 0);
         walk_row<PXL, OM0, REF, RECS>(Q.u.ent, n0, cnt, emax, x0, pxf, S);
         wave_sync();
       }
-      if (q == 0) resolve_lane<SMOOTH, OM, PXL, RECS>(D, so, F, lt, s, L, x0, j, S);
+      if (q == 0) resolve_lane<SMOOTH, OM, PXL, RECS>(D, so, F, lt, s, L, x0, j, S, PL);
     }
     jb0 += R;
   }
@@ -1030,9 +1104,10 @@ constexpr int rows_waves_per_simd() {
   return 4;
 }
 
-template <bool SMOOTH, int OM, int PXL, bool REF, int RECS>
+template <bool SMOOTH, int OM, int PXL, bool REF, int RECS, bool POSED>
 __global__ __launch_bounds__(RowsBlock<RECS>, (rows_waves_per_simd<SMOOTH, OM, PXL>())) void
-rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg) {
+rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg,
+            PoseArg<POSED> PZ) {
   constexpr int kBlock = RowsBlock<RECS>, kWaves = WavesPerScene<RECS>;
   __shared__ RowsLds<SMOOTH, OM, RECS> lds[2];
   __shared__ ListLds lists[kBlock / 64];
@@ -1041,13 +1116,49 @@ rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg)
   // from it - scene, LDS region, output bases) in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int s0 = (int)blockIdx.x * 2;
+  PoseLds* PL = nullptr;  // the workgroup's posed rows (POSED)
+  if constexpr (POSED) {
+    __shared__ PoseLds pose_lds;
+    PL = &pose_lds;
+    // pose phase: wave w evaluates pose w (the launch has at most one pose
+    // per wave; ips = 0 past the last), one lane per row of the two scenes,
+    // into LDS.  The pose's kernel arguments are read before anything else
+    // (one round of scalar loads).  The scenes' view-projections and the
+    // node table go to LDS too.
+    {
+      const int ips = PZ.ips[wave], first = PZ.first[wave];
+      const pbr_pose_desc P = PZ.d[wave];
+      if (lane < 2 * ips) {
+        const int psl = lane >= ips ? 1 : 0, i = lane - psl * ips, ps = s0 + psl;
+        if (ps < D.num_scenes) {
+          float4 m[3];
+          float R[9], sc;
+          pose_row(P, (int64_t)ps * ips + i, m, R, sc);
+          PoseRows& PR = pose_lds.sc[psl];
+          const int r = first + i;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) PR.m[r][q] = m[q];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) PR.rs[r][q] = R[q];
+          PR.rs[r][9] = sc;
+        }
+      }
+    }
+    if (tid < 8) {
+      const int ps = s0 + (tid >> 2);
+      if (ps < D.num_scenes)
+        pose_lds.vp[tid >> 2][tid & 3] =
+            reinterpret_cast<const float4*>(D.view_proj + (size_t)ps * 16)[tid & 3];
+    }
+    if (tid < D.num_nodes) pose_lds.node_row[tid] = (int8_t)PZ.node_row[tid];
+  }
   if (wave < 2 && lane == 0) lds[wave].any_clipped = 0;
   if (wave < 2 && lane == 1) {
     const float bgc[4] = {bg.x, bg.y, bg.z, bg.w};
     lds[wave].col[RECS] = pack_col<OM>(bgc);
   }
   __syncthreads();
-  if (F.vphase) {
+  if (POSED || F.vphase) {  // a posing launch is made only when the table fits
     // vertex phase: every thread takes vertex slots of the two scenes (the
     // table sits in the row-list space, which is not in use yet)
     VertRec* vt = reinterpret_cast<VertRec*>(&lists[0]);
@@ -1055,7 +1166,7 @@ rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg)
     const uint32_t nv1 = s0 + 1 < D.num_scenes ? scene_vertex_slots(D, s0 + 1) : 0u;
     for (uint32_t v = (uint32_t)tid; v < nv0 + nv1; v += (uint32_t)kBlock) {
       const int sv = v < nv0 ? 0 : 1;
-      vertex_slot(D, F, s0 + sv, sv ? v - nv0 : v, vt[v]);
+      vertex_slot(D, F, s0 + sv, sv ? v - nv0 : v, vt[v], PL);
     }
     __syncthreads();
     const int sl = tid / F.seg;
@@ -1063,7 +1174,7 @@ rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg)
     const int s = s0 + sl;
     if (sl < 2) {
       if (s < D.num_scenes)
-        setup_slot_v<SMOOTH, OM, RECS>(D, F, lt, s, slot, lds[sl], vt + (sl ? nv0 : 0u));
+        setup_slot_v<SMOOTH, OM, RECS>(D, F, lt, s, slot, lds[sl], vt + (sl ? nv0 : 0u), PL);
       else if (slot == 0)
         lds[sl].nrec = 0;
     }
@@ -1080,35 +1191,173 @@ rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg)
     }
   }
   __syncthreads();
+  if constexpr (POSED) {
+    // the posed rows' outputs (what pose_kernel stores), from the LDS rows
+    // of the vertex phase; after the last barrier, so no wave waits for them
+    if (tid < 2 * PZ.rows) {
+      const int psl = tid >= PZ.rows ? 1 : 0, r = tid - psl * PZ.rows, ps = s0 + psl;
+      if (ps < D.num_scenes) {
+        const PoseRows& P = PL->sc[psl];
+        const float4 m[3] = {P.m[r][0], P.m[r][1], P.m[r][2]};
+        float R[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) R[q] = P.rs[r][q];
+        const float sc = P.rs[r][9];
+        for (int k = 0; k < PZ.count; ++k)
+          if (r >= PZ.first[k] && r < PZ.first[k] + PZ.ips[k])
+            store_pose(PZ.d[k], (int64_t)ps * PZ.ips[k] + (r - PZ.first[k]), m, R, sc);
+      }
+    }
+  }
   // kWaves consecutive waves share scene s0 + w, each taking an equal share
   // of its rows
   const int sl = wave / kWaves, part = wave % kWaves;
   const int s = s0 + sl;
   if (s >= D.num_scenes) return;
   const int ja = (F.H * part) / kWaves, jb = (F.H * (part + 1)) / kWaves;
-  raster_scene<SMOOTH, OM, PXL, REF, RECS>(D, O, F, lt, s, lds[sl], lists[wave], lane, ja, jb);
+  raster_scene<SMOOTH, OM, PXL, REF, RECS>(D, O, F, lt, s, lds[sl], lists[wave], lane, ja, jb,
+                                           PL);
 }
 
-template <bool SMOOTH, int OM, bool REF, int RECS>
+template <bool SMOOTH, int OM, bool REF, int RECS, bool POSED>
 hipError_t launch_rows_t(const pbr_scene_desc* D, const pbr_frame_out* O, const RowsFrame& F,
-                         const Light& lt, float4 bg, hipStream_t st) {
+                         const Light& lt, float4 bg, const PoseArg<POSED>& pz, hipStream_t st) {
   const int grid = (D->num_scenes + 1) / 2;
   // the 32-record form runs tiles of at most 64 columns (4-pixel lanes)
   if (RECS == 32 || D->width <= 64)
-    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 4, REF, RECS>), dim3(grid), dim3(RowsBlock<RECS>), 0,
-                       st, *D, *O, F, lt, bg);
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 4, REF, RECS, POSED>), dim3(grid),
+                       dim3(RowsBlock<RECS>), 0, st, *D, *O, F, lt, bg, pz);
   else if constexpr (RECS == 64)
-    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 8, REF, RECS>), dim3(grid), dim3(RowsBlock<RECS>), 0,
-                       st, *D, *O, F, lt, bg);
+    hipLaunchKernelGGL((rows_kernel<SMOOTH, OM, 8, REF, RECS, POSED>), dim3(grid),
+                       dim3(RowsBlock<RECS>), 0, st, *D, *O, F, lt, bg, pz);
   return hipGetLastError();
 }
 
 // Flat scenes: the 32-record form when `small`, else the 64-record one.
-template <int OM, bool REF>
+template <int OM, bool REF, bool POSED>
 hipError_t launch_rows_flat(const pbr_scene_desc* D, const pbr_frame_out* O, const RowsFrame& F,
-                            const Light& lt, float4 bg, bool small, hipStream_t st) {
-  return small ? launch_rows_t<false, OM, REF, 32>(D, O, F, lt, bg, st)
-               : launch_rows_t<false, OM, REF, 64>(D, O, F, lt, bg, st);
+                            const Light& lt, float4 bg, bool small, const PoseArg<POSED>& pz,
+                            hipStream_t st) {
+  return small ? launch_rows_t<false, OM, REF, 32, POSED>(D, O, F, lt, bg, pz, st)
+               : launch_rows_t<false, OM, REF, 64, POSED>(D, O, F, lt, bg, pz, st);
+}
+
+// The instantiation for the output dtype and shading.
+template <bool POSED>
+hipError_t launch_rows_for(const pbr_scene_desc* D, const pbr_frame_out* O, const RowsFrame& F,
+                           const Light& lt, float4 bg, bool smooth, bool small,
+                           const PoseArg<POSED>& pz, hipStream_t st) {
+  switch (O->dtype) {
+    case PBR_DTYPE_U8:
+      if (smooth) return launch_rows_t<true, 0, true, 64, POSED>(D, O, F, lt, bg, pz, st);
+      if (O->vis) return launch_rows_flat<0, true, POSED>(D, O, F, lt, bg, small, pz, st);
+      return launch_rows_flat<0, false, POSED>(D, O, F, lt, bg, small, pz, st);
+    case PBR_DTYPE_BF16:
+      return smooth ? launch_rows_t<true, 1, true, 64, POSED>(D, O, F, lt, bg, pz, st)
+                    : launch_rows_flat<1, true, POSED>(D, O, F, lt, bg, small, pz, st);
+    default:
+      return smooth ? launch_rows_t<true, 2, true, 64, POSED>(D, O, F, lt, bg, pz, st)
+                    : launch_rows_flat<2, true, POSED>(D, O, F, lt, bg, small, pz, st);
+  }
+}
+
+bool overlap(const void* a, size_t na, const void* b, size_t nb) {
+  if (!a || !b || !na || !nb) return false;
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + nb && y < x + na;
+}
+
+// Flat scenes of at most 32 slots and 64 columns run the 32-record form.
+bool rows_small(const pbr_scene_desc* D) {
+  return !any_smooth(D) && max_slots(D) <= 32 && D->width <= 64;
+}
+
+// Whether both scenes' vertex slots fit the LDS vertex table (the vertex
+// phase transforms each vertex once).
+bool rows_vphase(const pbr_scene_desc* D) {
+  uint64_t vslots = 0;
+  for (int j = 0; j < D->num_nodes; ++j)
+    vslots += (uint64_t)D->nodes[j].instances_per_scene *
+              (uint64_t)D->meshes[D->nodes[j].mesh].num_vertices;
+  return 2 * vslots <= (uint64_t)(rows_small(D) ? VertCap<32> : VertCap<64>);
+}
+
+// The pose table of one launch, or false if the launch cannot apply the
+// poses itself: the vertex phase must run (the posed rows are evaluated
+// there), every pose with rows must write the transforms of its own
+// unshared node of D (same pointer, rows = num_scenes * ips, no per-scene
+// instance counts, at least one vertex), at most kFusedRows posed rows per
+// scene, and no pose output may overlap another pose's outputs or anything
+// the launch reads (node transforms / colours / counts, the
+// view-projections) other than the posed node's own transforms.
+bool match_poses(const pbr_scene_desc* D, const pbr_pose_desc* poses, int32_t count,
+                 RowsPoses* T) {
+  if (count < 0 || count > kMaxPoseBatch || (count > 0 && !poses)) return false;
+  if (!rows_vphase(D)) return false;
+  const int64_t N = D->num_scenes;
+  int node_of[kMaxPoseBatch];
+  RowsPoses t;
+  memset(&t, 0, sizeof(t));
+  memset(t.node_row, 0xff, sizeof(t.node_row));
+  uint32_t used = 0;
+  for (int k = 0; k < count; ++k) {
+    const pbr_pose_desc& p = poses[k];
+    if (p.rows == 0) continue;
+    int node = -1;
+    for (int j = 0; j < D->num_nodes; ++j) {
+      if (D->nodes[j].transforms != p.transforms) continue;
+      if (node >= 0) return false;  // two nodes draw this buffer
+      node = j;
+    }
+    if (node < 0 || ((used >> node) & 1u)) return false;
+    const pbr_node& nd = D->nodes[node];
+    if (nd.shared || nd.counts || nd.instances_per_scene < 1 ||
+        D->meshes[nd.mesh].num_vertices < 1 || (int64_t)p.rows != N * nd.instances_per_scene)
+      return false;
+    if (t.count >= kFusedPoses) return false;
+    used |= 1u << node;
+    node_of[t.count] = node;
+    t.node_row[node] = t.rows;
+    t.ips[t.count] = nd.instances_per_scene;
+    t.first[t.count] = t.rows;
+    t.d[t.count] = p;
+    t.rows += nd.instances_per_scene;
+    ++t.count;
+    if (t.rows > kFusedRows) return false;
+  }
+  // one pose per wave: two in the 32-record form, four in the 64-record one
+  if (t.count > (rows_small(D) ? 2 : 4)) return false;
+  for (int k = 0; k < t.count; ++k) {
+    const pbr_pose_desc& p = t.d[k];
+    const size_t rows = (size_t)p.rows;
+    const void* out[3] = {p.transforms, p.rot3, p.scale};
+    const size_t out_n[3] = {rows * 64, p.rot3 ? rows * 36 : 0, p.scale ? rows * 4 : 0};
+    for (int a = 0; a < 3; ++a) {
+      if (overlap(out[a], out_n[a], D->view_proj, (size_t)N * 64)) return false;
+      for (int j = 0; j < D->num_nodes; ++j) {
+        const pbr_node& nd = D->nodes[j];
+        const size_t nrows = (size_t)(nd.shared ? 1 : N) * (size_t)nd.instances_per_scene;
+        if (!(a == 0 && j == node_of[k]) && overlap(out[a], out_n[a], nd.transforms, nrows * 64))
+          return false;
+        if (overlap(out[a], out_n[a], nd.colors, nrows * 16) ||
+            overlap(out[a], out_n[a], nd.counts, nd.counts ? (size_t)N * 4 : 0))
+          return false;
+      }
+      for (int k2 = k + 1; k2 < t.count; ++k2) {
+        const pbr_pose_desc& q = t.d[k2];
+        const size_t qr = (size_t)q.rows;
+        if (overlap(out[a], out_n[a], q.transforms, qr * 64) ||
+            overlap(out[a], out_n[a], q.rot3, q.rot3 ? qr * 36 : 0) ||
+            overlap(out[a], out_n[a], q.scale, q.scale ? qr * 4 : 0))
+          return false;
+      }
+    }
+    for (int a = 1; a < 3; ++a)
+      for (int b = 0; b < a; ++b)
+        if (overlap(out[a], out_n[a], out[b], out_n[b])) return false;
+  }
+  if (T) *T = t;
+  return true;
 }
 
 }  // namespace
@@ -1119,8 +1368,12 @@ bool rows_supported(const pbr_scene_desc* D) {
   return max_slots(D) <= 64 && D->width <= 128 && D->height <= 16384;
 }
 
+bool rows_can_pose(const pbr_scene_desc* D, const pbr_pose_desc* poses, int32_t count) {
+  return match_poses(D, poses, count, nullptr);
+}
+
 int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_render_opts* opts,
-                hipStream_t st) {
+                hipStream_t st, const pbr_pose_desc* poses, int32_t npose) {
   if (max_slots(D) > 64)
     return fail(PBR_ERR_UNSUPPORTED, "row-span kernel needs at most 64 triangle slots per scene");
   if (D->width > 128 || D->height > 16384)
@@ -1136,37 +1389,19 @@ int launch_rows(const pbr_scene_desc* D, const pbr_frame_out* O, const pbr_rende
   if (cap != 0 && (cap < 8 || cap > kCap))
     return fail(PBR_ERR_INVALID, "row_list_cap %d outside [8, %d]", cap, kCap);
   const bool smooth = any_smooth(D);
-  // flat scenes of at most 32 slots and 64 columns run the 32-record form
-  const bool small = !smooth && slots <= 32 && D->width <= 64;
+  const bool small = rows_small(D);
   F.cap = cap ? cap : kCap;
-  // the vertex phase needs both scenes' vertex slots in the table
-  uint64_t vslots = 0;
-  for (int j = 0; j < D->num_nodes; ++j)
-    vslots += (uint64_t)D->nodes[j].instances_per_scene *
-              (uint64_t)D->meshes[D->nodes[j].mesh].num_vertices;
-  F.vphase = (2 * vslots <= (uint64_t)(small ? VertCap<32> : VertCap<64>)) ? 1 : 0;
+  F.vphase = rows_vphase(D) ? 1 : 0;
   const Light lt = host_light(D);
   const float4 bg = make_float4(D->background[0], D->background[1], D->background[2],
                                 D->background[3]);
-  hipError_t e;
-  switch (O->dtype) {
-    case PBR_DTYPE_U8:
-      if (smooth)
-        e = launch_rows_t<true, 0, true, 64>(D, O, F, lt, bg, st);
-      else if (O->vis)
-        e = launch_rows_flat<0, true>(D, O, F, lt, bg, small, st);
-      else
-        e = launch_rows_flat<0, false>(D, O, F, lt, bg, small, st);
-      break;
-    case PBR_DTYPE_BF16:
-      e = smooth ? launch_rows_t<true, 1, true, 64>(D, O, F, lt, bg, st)
-                 : launch_rows_flat<1, true>(D, O, F, lt, bg, small, st);
-      break;
-    default:
-      e = smooth ? launch_rows_t<true, 2, true, 64>(D, O, F, lt, bg, st)
-                 : launch_rows_flat<2, true>(D, O, F, lt, bg, small, st);
-      break;
-  }
+  RowsPoses pz;
+  if (npose > 0 && !match_poses(D, poses, npose, &pz))
+    return fail(PBR_ERR_UNSUPPORTED, "the row-span launch cannot apply these poses itself");
+  const hipError_t e =
+      npose > 0 && pz.count > 0
+          ? launch_rows_for<true>(D, O, F, lt, bg, smooth, small, pz, st)
+          : launch_rows_for<false>(D, O, F, lt, bg, smooth, small, NoPoses{}, st);
   if (e != hipSuccess) return fail(PBR_ERR_LAUNCH, "row-span launch failed: %s", hipGetErrorString(e));
   return PBR_OK;
 }

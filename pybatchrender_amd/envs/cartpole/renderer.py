@@ -44,6 +44,9 @@ def _state_rows(state_batch, n: int, device) -> torch.Tensor:
 
 
 class CartPoleRenderer(PBRRenderer):
+    # _step only poses nodes: step() poses and renders in one launch
+    fuse_poses = True
+
     def __init__(self, cfg: PBRConfig | dict | None = None):
         super().__init__(cfg)
         ips = 1
@@ -116,6 +119,9 @@ class CartPoleRenderer(PBRRenderer):
 
 
 class CartPoleGroundRenderer(PBRRenderer):
+    # _step only poses nodes: step() poses and renders in one launch
+    fuse_poses = True
+
     CART_SCALE = (0.4, 0.2, 0.2)
     POLE_SCALE = (0.05, 0.05, 1.0)
     GROUND_SIZE = 10.0

@@ -41,6 +41,10 @@ def resolve_model(path: str):
 
 
 class SteeringRenderer(PBRRenderer):
+    # _step reads no posed node's buffers after posing (it sets colours and
+    # the camera): step() poses and renders in one launch
+    fuse_poses = True
+
     def __init__(self, cfg: PBRConfig | dict | None = None) -> None:
         super().__init__(cfg)
         n = int(self.cfg.num_scenes)

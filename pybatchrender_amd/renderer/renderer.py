@@ -61,6 +61,8 @@ class PBRRenderer:
         self._desc_cache = None
         self._opts_cache = None
         self._ws_cache = None
+        # poses set_poses recorded for the next render (fuse_poses steps)
+        self._pending_poses: list | None = None
 
     # --- scene construction --------------------------------------------------
     def set_background_color(self, r: float, g: float, b: float, a: float = 1.0) -> None:
@@ -310,7 +312,18 @@ class PBRRenderer:
             if self._workspace is None or self._workspace.numel() < ws_bytes:
                 self._workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
             ws_ptr = self._workspace.data_ptr()
-        _native.render(desc, fo, opts, ws_ptr, ws_bytes, _native.stream_ptr(self.device))
+        stream = _native.stream_ptr(self.device)
+        pending = self._pending_poses
+        if pending:
+            self._pending_poses = None
+            descs = [p[0] for _, p in pending]
+            head, tail = descs[:-8], descs[-8:]  # one launch poses at most 8 nodes
+            for k in range(0, len(head), 8):
+                _native.set_poses(head[k:k + 8], stream)
+            _native.render_posed(desc, fo, opts, tail, ws_ptr, ws_bytes, stream)
+            del pending
+        else:
+            _native.render(desc, fo, opts, ws_ptr, ws_bytes, stream)
         del keep
         result = {"pixels": out}
         if isinstance(depth, torch.Tensor):
@@ -404,6 +417,13 @@ class PBRRenderer:
                     for node, kw in poses]
         prepared = [(n, p) for n, p in prepared if p is not None]
         if self.device.type == "cuda":
+            if self._pending_poses is not None:
+                # inside a fuse_poses step: the render applies them; a node
+                # posed again keeps only its last pose (each pose writes
+                # every row of the node)
+                last = {id(n): (n, p) for n, p in self._pending_poses + prepared}
+                self._pending_poses = list(last.values())
+                return
             stream = _native.stream_ptr(self.device)
             for k in range(0, len(prepared), 8):
                 _native.set_poses([p[0] for _, p in prepared[k:k + 8]], stream)
@@ -415,11 +435,39 @@ class PBRRenderer:
         """Subclass hook: map the caller's state into node/camera tensors."""
         return None
 
+    # Subclasses whose ``_step`` changes the scene only through set_poses
+    # (and reads no posed node's buffers afterwards) set this: step() then
+    # hands the poses to the render, which applies them in its own launch
+    # when the row-span kernel draws the batch (pbr_render_posed).
+    fuse_poses: bool = False
+
+    def _flush_poses(self) -> None:
+        """Launch the poses recorded for the next render, if any."""
+        pending, self._pending_poses = self._pending_poses, None
+        if pending:
+            stream = _native.stream_ptr(self.device)
+            descs = [p[0] for _, p in pending]
+            for k in range(0, len(descs), 8):
+                _native.set_poses(descs[k:k + 8], stream)
+
+    def _step_and_render(self, args: tuple, kwargs: dict, **render_kw) -> dict:
+        """``_step(*args, **kwargs)`` then ``render(**render_kw)``; with
+        fuse_poses the poses ``_step`` sets are applied by the render call."""
+        if not (self.fuse_poses and self.device.type == "cuda"):
+            self._step(*args, **kwargs)
+            return self.render(**render_kw)
+        self._pending_poses = []
+        try:
+            self._step(*args, **kwargs)
+            return self.render(**render_kw)
+        finally:
+            self._flush_poses()  # what an error left unapplied
+
     def step(self, *args, return_pixels: bool = True, **kwargs):
         """renderer.py:377-389: update the scene via ``_step``, then render."""
-        self._step(*args, **kwargs)
         if return_pixels:
-            return self.render()["pixels"]
+            return self._step_and_render(args, kwargs)["pixels"]
+        self._step(*args, **kwargs)
         return None
 
     def __call__(self, *args, return_pixels: bool = True, **kwargs):
@@ -512,10 +560,12 @@ class GraphedStep:
                         if n.instance_counts is not None]
 
     def _body(self) -> None:
+        kw = dict(out=self.out, depth=self.depth if self.depth is not None else False,
+                  vis=self.vis if self.vis is not None else False, workspace=self.workspace)
         if self.state is not None:
-            self.r._step(self.state)
-        self.r.render(out=self.out, depth=self.depth if self.depth is not None else False,
-                      vis=self.vis if self.vis is not None else False, workspace=self.workspace)
+            self.r._step_and_render((self.state,), {}, **kw)
+        else:
+            self.r.render(**kw)
 
     def outputs(self) -> dict:
         res = {"pixels": self.out}
