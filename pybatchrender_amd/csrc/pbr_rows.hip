@@ -1097,15 +1097,16 @@ __device__ __forceinline__ void raster_scene(const pbr_scene_desc& D, const pbr_
 // Waves per SIMD an instantiation is built for (the second launch bound;
 // what its LDS allows): four (the 32-record form: eight two-wave workgroups
 // per CU; the 64-record form: four four-wave ones), smooth forms with bf16 /
-// float32 colours three, or two with 8-pixel lanes.
-template <bool SMOOTH, int OM, int PXL>
+// float32 colours, and posing smooth forms (their pose rows take the LDS
+// of a fourth workgroup), three, or two with 8-pixel lanes.
+template <bool SMOOTH, int OM, int PXL, bool POSED>
 constexpr int rows_waves_per_simd() {
-  if (SMOOTH && OM != 0) return PXL == 8 ? 2 : 3;
+  if (SMOOTH && (OM != 0 || POSED)) return PXL == 8 ? 2 : 3;
   return 4;
 }
 
 template <bool SMOOTH, int OM, int PXL, bool REF, int RECS, bool POSED>
-__global__ __launch_bounds__(RowsBlock<RECS>, (rows_waves_per_simd<SMOOTH, OM, PXL>())) void
+__global__ __launch_bounds__(RowsBlock<RECS>, (rows_waves_per_simd<SMOOTH, OM, PXL, POSED>())) void
 rows_kernel(pbr_scene_desc D, pbr_frame_out O, RowsFrame F, Light lt, float4 bg,
             PoseArg<POSED> PZ) {
   constexpr int kBlock = RowsBlock<RECS>, kWaves = WavesPerScene<RECS>;
