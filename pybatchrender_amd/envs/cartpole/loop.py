@@ -89,7 +89,8 @@ class CartPoleLoop:
             if desc.rows != N:
                 raise ValueError(f"node {node.name}: {desc.rows} pose rows for {N} scenes")
             self.poses.append(desc)
-            self._pose_keep.append(keep)
+            # the sources and the output buffers the descriptor points to
+            self._pose_keep.append((keep, node.transforms_b44, node.rot3_b33, node.scale_b11))
         if len(self.poses) > 8:
             raise ValueError("at most 8 posed nodes")
         Nn, C, H, W = renderer.output_shape()

@@ -551,10 +551,15 @@ class GraphedStep:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self._body()
-        # what the graph reads: the descriptor's tensors (geometry, node
-        # buffers, view-projection, counts); held so they outlive renderer
-        # changes
+        # what the graph reads and writes: the descriptor's tensors
+        # (geometry, node buffers, view-projection, counts) and the pose
+        # outputs of every node (rot3 / scale are not in the descriptor);
+        # held so they outlive renderer changes
         self._keep = list(renderer._desc_cache[2])
+        for n in renderer._drawn_nodes():
+            self._keep += [t for t in (n.transforms_b44, getattr(n, "rot3_b33", None),
+                                       getattr(n, "scale_b11", None))
+                           if isinstance(t, torch.Tensor)]
         self._budget = renderer.slot_budget()
         self._counts = [(n, n._counts_version) for n in renderer._drawn_nodes()
                         if n.instance_counts is not None]
