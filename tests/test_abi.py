@@ -166,3 +166,28 @@ def test_slot_budget_follows_in_place_count_edits():
     assert b2 < b1
     r.scene_descriptor()
     assert r._cached_opts().slot_budget == b2
+
+
+def test_render_posed_validation_without_gpu():
+    """pbr_render_posed checks the scene, the poses and the options before it
+    enqueues anything (pbr_set_poses + pbr_render's errors, in that order)."""
+    lib = _native.library()
+    d = _abi.pbr_scene_desc()
+    d.num_scenes, d.width, d.height = 1, 0, 64
+    fo = _abi.pbr_frame_out()
+    fo.channels, fo.dtype = 3, _abi.PBR_DTYPE_U8
+    rc = lib.pbr_render_posed(ctypes.byref(d), ctypes.byref(fo), None, None, 0, None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "resolution" in _native.last_error()
+    d.width = 64
+    fo.color = d.view_proj = 64  # never dereferenced: validation fails first
+    poses = (_abi.pbr_pose_desc * 9)()
+    rc = lib.pbr_render_posed(ctypes.byref(d), ctypes.byref(fo), None, poses, 9, None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "pose count" in _native.last_error()
+    poses[0].rows, poses[0].transforms = 1, 8  # not 16-byte aligned
+    rc = lib.pbr_render_posed(ctypes.byref(d), ctypes.byref(fo), None, poses, 1, None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "aligned" in _native.last_error()
+    opts = _abi.pbr_render_opts()
+    opts.kernel = 9
+    rc = lib.pbr_render_posed(ctypes.byref(d), ctypes.byref(fo), ctypes.byref(opts), None, 0,
+                              None, 0, None)
+    assert rc == _abi.PBR_ERR_INVALID and "unknown kernel" in _native.last_error()
