@@ -94,7 +94,7 @@ class _PosedScene(PBRRenderer):
     [N, 7 * (ips)] state, over a shared ground quad."""
     fuse_poses = True
 
-    def __init__(self, n, res, sphere: bool, **cfg):
+    def __init__(self, n, res, sphere: bool, panels: int = 0, **cfg):
         super().__init__(PBRConfig(num_scenes=n, tile_resolution=res, device="cuda",
                                    kernel="rows", **cfg))
         gen = torch.Generator("cpu").manual_seed(40)
@@ -110,6 +110,12 @@ class _PosedScene(PBRRenderer):
             from pybatchrender.renderer import mesh
             self.sphere = self.add_node(mesh.uv_sphere(segments=6, rings=4), instances_per_scene=1)
             self.sphere.set_colors(torch.rand(n, 4, generator=gen))
+        self.panels = []  # more posed nodes: unshared quads, one per scene
+        for _ in range(panels):
+            q = self.add_node("models/quad", instances_per_scene=1,
+                              model_pivot_relative_point=(0.5, 0.5, 0.0))
+            q.set_colors(torch.rand(n, 4, generator=gen))
+            self.panels.append(q)
         self.add_camera(fov_y_deg=70.0, z_near=0.4, z_far=60.0)
         self._pbr_cam.set_positions_and_lookat(torch.zeros(3), torch.tensor([0.2, 1.0, 0.1]))
         self.add_light()
@@ -122,6 +128,8 @@ class _PosedScene(PBRRenderer):
         poses = [(self.boxes, {k: b[:, i] for i, k in enumerate("xyzhprs")})]
         if self.sphere is not None:
             poses.append((self.sphere, dict(x=st[:, 7], y=st[:, 8], z=st[:, 9], s=st[:, 10])))
+        for k, q in enumerate(self.panels):
+            poses.append((q, dict(x=st[:, 9] + k, y=st[:, 8] + 1.0, p=st[:, 3], h=1.2, s=2.0)))
         self.set_poses(poses)
 
 
@@ -163,6 +171,24 @@ def test_posed_boxes_and_spheres(cuda, sphere, dtype, near):
     # depth and visibility too (the forms that keep records per pixel)
     got = fused._step_and_render((states[0],), {}, depth=True, vis=True)
     check_parity(fused, label + " depth/vis", got=got)
+
+
+@pytest.mark.parametrize("sphere", [False, True])
+def test_three_posed_nodes(cuda, sphere):
+    """Three poses: one launch in the 64-record form (four waves, one pose
+    each; boxes + sphere + a panel + ground = 64 slots), the two launches in
+    the 32-record form (two waves; boxes + two panels + ground)."""
+    n = 21
+    panels = 1 if sphere else 2
+    fused = _PosedScene(n, (40, 48), sphere, panels=panels)
+    plain = _PosedScene(n, (40, 48), sphere, panels=panels)
+    plain.fuse_poses = False
+    states = [_posed_states(n, seed, False).cuda() for seed in (52, 53)]
+    _compare_steps(fused, plain, states, f"three posed nodes sphere={sphere}")
+    names = _kernel_names(lambda: fused.step(states[0]))
+    print(f"[posed] three poses, sphere={sphere}: {names}", flush=True)
+    if names:
+        assert any("pose_kernel" in k for k in names) == (not sphere)
 
 
 def test_graph_replays_of_fused_step(cuda):
