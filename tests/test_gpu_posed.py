@@ -94,7 +94,7 @@ class _PosedScene(PBRRenderer):
     [N, 7 * (ips)] state, over a shared ground quad."""
     fuse_poses = True
 
-    def __init__(self, n, res, sphere: bool, panels: int = 0, **cfg):
+    def __init__(self, n, res, sphere: bool, panels: int = 0, boxes: int = 2, **cfg):
         super().__init__(PBRConfig(num_scenes=n, tile_resolution=res, device="cuda",
                                    kernel="rows", **cfg))
         gen = torch.Generator("cpu").manual_seed(40)
@@ -102,9 +102,10 @@ class _PosedScene(PBRRenderer):
                                model_scale=(40.0, 40.0, 1.0),
                                model_pivot_relative_point=(0.5, 0.5, 0.0))
         ground.set_colors(torch.tensor([[0.3, 0.6, 0.3, 1.0]]))
-        self.boxes = self.add_node("models/box", instances_per_scene=2,
+        self.nbox = boxes
+        self.boxes = self.add_node("models/box", instances_per_scene=boxes,
                                    model_pivot_relative_point=(0.5, 0.5, 0.5))
-        self.boxes.set_colors(torch.rand(n * 2, 4, generator=gen))
+        self.boxes.set_colors(torch.rand(n * boxes, 4, generator=gen))
         self.sphere = None
         if sphere:
             from pybatchrender.renderer import mesh
@@ -123,8 +124,9 @@ class _PosedScene(PBRRenderer):
 
     def _step(self, st: torch.Tensor) -> None:
         n = self.num_scenes
-        b = st[:, :7].reshape(n, 7).repeat_interleave(2, 0).contiguous()
-        b[1::2, 0] += 1.5
+        b = st[:, :7].reshape(n, 7).repeat_interleave(self.nbox, 0).contiguous()
+        for k in range(1, self.nbox):
+            b[k::self.nbox, 0] += 1.5 * k
         poses = [(self.boxes, {k: b[:, i] for i, k in enumerate("xyzhprs")})]
         if self.sphere is not None:
             poses.append((self.sphere, dict(x=st[:, 7], y=st[:, 8], z=st[:, 9], s=st[:, 10])))
@@ -171,6 +173,21 @@ def test_posed_boxes_and_spheres(cuda, sphere, dtype, near):
     # depth and visibility too (the forms that keep records per pixel)
     got = fused._step_and_render((states[0],), {}, depth=True, vis=True)
     check_parity(fused, label + " depth/vis", got=got)
+
+
+@pytest.mark.parametrize("res", [(48, 48), (96, 40)])
+def test_posed_flat_64_record_form(cuda, res):
+    """Flat scenes over 32 slots (three boxes per scene + ground = 38) run the
+    64-record form, with 4-pixel lanes up to 64 columns and 8-pixel lanes
+    past them; near-plane crossings included."""
+    n = 19
+    fused = _PosedScene(n, res, False, boxes=3)
+    plain = _PosedScene(n, res, False, boxes=3)
+    plain.fuse_poses = False
+    states = [_posed_states(n, 54, True).cuda(), _posed_states(n, 55, False).cuda()]
+    _compare_steps(fused, plain, states, f"posed flat 64-record form {res}")
+    got = fused._step_and_render((states[0],), {}, depth=True, vis=True)
+    check_parity(fused, f"posed flat 64-record form {res} depth/vis", got=got)
 
 
 @pytest.mark.parametrize("sphere", [False, True])
