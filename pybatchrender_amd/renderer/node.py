@@ -27,6 +27,10 @@ from .mesh import Mesh, bake, load_model
 from .shader_context import PBRShaderContext
 
 
+# Pose components a node takes when a pose leaves them out.
+_POSE_DEFAULTS = {"x": 0.0, "y": 0.0, "z": 0.0, "h": 0.0, "p": 0.0, "r": 0.0, "s": 1.0}
+
+
 class PBRNode(PBRShaderContext):
     def __init__(
         self,
@@ -183,11 +187,13 @@ class PBRNode(PBRShaderContext):
 
     def _pose_inputs(self, **comps):
         """Output buffers sized for the pose, and the pose descriptor (GPU) or
-        the full per-row component tensors (CPU)."""
+        the full per-row component tensors (CPU).  Components not given take
+        the identity pose's values (0, and 1 for s)."""
         if not self.has_geometry:
             return None
         if self.device.type == "cuda":
             return self._pose_inputs_gpu(comps)
+        comps = {**_POSE_DEFAULTS, **comps}
         B = self.buf_instances
         dev = self.device
         if self.transforms_b44.shape != (B, 4, 4) or not self.transforms_b44.is_contiguous():
@@ -246,7 +252,8 @@ class PBRNode(PBRShaderContext):
                 self.rot3_b33 = torch.empty((B, 3, 3), dtype=torch.float32, device=dev)
             if self.scale_b11.shape != (B, 1, 1) or not self.scale_b11.is_contiguous():
                 self.scale_b11 = torch.empty((B, 1, 1), dtype=torch.float32, device=dev)
-            t = _abi.pbr_pose_desc()
+            t = _abi.pbr_pose_desc()  # the identity pose: components 0, s = 1
+            t.s.value = 1.0
             t.transforms = self.transforms_b44.data_ptr()
             t.rot3 = self.rot3_b33.data_ptr()
             t.scale = self.scale_b11.data_ptr()

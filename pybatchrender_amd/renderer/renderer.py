@@ -412,9 +412,7 @@ class PBRRenderer:
     def set_poses(self, poses) -> None:
         """PBRNode.set_pose for several nodes, one HIP launch on the GPU.
         ``poses`` is a sequence of (node, {x, y, z, h, p, r, s}) pairs."""
-        prepared = [(node, node._pose_inputs(**{"x": 0.0, "y": 0.0, "z": 0.0, "h": 0.0, "p": 0.0,
-                                                "r": 0.0, "s": 1.0, **kw}))
-                    for node, kw in poses]
+        prepared = [(node, node._pose_inputs(**kw)) for node, kw in poses]
         prepared = [(n, p) for n, p in prepared if p is not None]
         if self.device.type == "cuda":
             if self._pending_poses is not None:
