@@ -62,6 +62,19 @@ def test_cartpole_step_fused_matches_two_launches(cuda, n):
     _compare_steps(fused, plain, states, f"cartpole ground N={n}")
 
 
+@pytest.mark.parametrize("dtype,depth", [("uint8", False), ("bfloat16", True)])
+def test_cartpole_step_fused_at_16384_scenes(cuda, dtype, depth):
+    """The largest CartPole batch the benchmarks run (the env-loop size), in
+    the headline format and the env loop's bf16 + depth format."""
+    n = 16384
+    cfg = dict(num_scenes=n, device="cuda", output_dtype=dtype, output_depth=depth)
+    fused = CartPoleGroundRenderer(CartPoleConfig(**cfg))
+    plain = CartPoleGroundRenderer(CartPoleConfig(**cfg))
+    plain.fuse_poses = False
+    states = [ground_states(n, seed).cuda() for seed in (60, 61)]
+    _compare_steps(fused, plain, states, f"cartpole ground N={n} {dtype} depth={depth}")
+
+
 def test_cartpole_reference_scene_step_fused(cuda):
     n = 130
     fused = CartPoleRenderer(CartPoleConfig(num_scenes=n, device="cuda"))
