@@ -268,8 +268,9 @@ def _abi_call(r: PBRRenderer, poses: list, fused: bool) -> torch.Tensor:
 
 def test_render_posed_cases_it_does_not_fuse(cuda):
     """Poses the row-span launch does not apply itself (a shared node, a
-    node of another scene, a pose of fewer rows than the node holds, the
-    tile path) take the two launches: same frames and rows as calling them."""
+    node of another scene, a pose of fewer rows than the node holds, a node
+    drawn with per-scene instance counts, the tile path) take the two
+    launches: same frames and rows as calling them."""
     n = 40
     gen = torch.Generator("cpu").manual_seed(41)
     mk = lambda: _PosedScene(n, (32, 32), False)  # noqa: E731
@@ -312,6 +313,15 @@ def test_render_posed_cases_it_does_not_fuse(cuda):
         del pd
     assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
     del keep_a, keep_b, desc_a
+    # a posed node drawn with per-scene instance counts
+    cnt = torch.randint(0, 3, (n,), generator=gen)
+    c_a, c_b = mk(), mk()
+    c_a.boxes.set_instance_counts(cnt)
+    c_b.boxes.set_instance_counts(cnt)
+    xb = (torch.rand(2 * n, generator=gen) * 2 - 1).cuda()
+    f1 = _abi_call(c_a, [(c_a.boxes, dict(x=xb, y=4.5, h=0.7))], fused=True)
+    f2 = _abi_call(c_b, [(c_b.boxes, dict(x=xb, y=4.5, h=0.7))], fused=False)
+    assert torch.equal(f1, f2) and torch.equal(c_a.boxes.transforms_b44, c_b.boxes.transforms_b44)
     # the tile path: 8 boxes per scene (96 slots)
     t_a = workloads.many_cubes(num_scenes=8, resolution=64, instances=8, seed=3,
                                output_dtype="uint8", output_depth=False)
